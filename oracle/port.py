@@ -1,0 +1,150 @@
+"""ctypes binding of the C restatement (oracle/qv_oracle.c -> oracle/_ref/liboracle.so).
+
+Test infrastructure only (see oracle/__init__.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "liboracle.so")
+
+VEL_BS, VEL_TREE = 1, 2
+BC = {"open": 0, "periodic": 1, "openx": 2, "openxy": 3}
+NODE_DOUBLES = 22
+
+
+class QvoParams(C.Structure):
+    _fields_ = [("quant_circ", C.c_double), ("corea", C.c_double), ("box_size", C.c_double),
+                ("delta", C.c_double), ("tree_theta", C.c_double), ("dt", C.c_double),
+                ("velocity", C.c_int32), ("boundary", C.c_int32),
+                ("tree_extra_correction", C.c_int32), ("deriv_order", C.c_int32)]
+
+
+def make_params(*, quant_circ=9.97e-4, corea=8.244023e-9, box_size=0.1, delta=5e-4,
+                tree_theta=0.0, dt=1e-6, velocity="Tree", boundary="periodic",
+                tree_extra_correction=True, deriv_order=2) -> QvoParams:
+    return QvoParams(quant_circ, corea, box_size, delta, tree_theta, dt,
+                     VEL_TREE if velocity == "Tree" else VEL_BS, BC[boundary],
+                     1 if tree_extra_correction else 0, int(deriv_order))
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "port"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        d, i32, i64 = P(C.c_double), P(C.c_int32), P(C.c_int64)
+        L.qvo_velocity.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, i64, C.c_int32]
+        L.qvo_velocity.restype = C.c_int
+        L.qvo_velocity_targets.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, i32, C.c_int32,
+                                           d, i64, C.c_int32, d, d]
+        L.qvo_velocity_targets.restype = C.c_int
+        L.qvo_tree_dump.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, C.c_int64]
+        L.qvo_tree_dump.restype = C.c_int64
+        L.qvo_ghostp.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, d]
+        L.qvo_ghostp.restype = C.c_int
+        L.qvo_ab_update.argtypes = [P(QvoParams), C.c_int32, d, i32, d, d, d, d]
+        L.qvo_ab_update.restype = C.c_int
+        L.qvo_run.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, C.c_int32, d, C.c_int32]
+        L.qvo_run.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _prep(fil):
+    x = np.ascontiguousarray(fil.x, dtype=np.float64)
+    inf = np.ascontiguousarray(fil.infront, dtype=np.int32)
+    beh = np.ascontiguousarray(fil.behind, dtype=np.int32)
+    return x, inf, beh
+
+
+def velocity(params: QvoParams, fil, nthreads: int = 0):
+    """(u, u_li, u_bs, evals) for every slot, as calc_velocity computes them."""
+    x, inf, beh = _prep(fil)
+    n = x.shape[0]
+    u = np.zeros((n, 3)); uli = np.zeros((n, 3)); ubs = np.zeros((n, 3))
+    ev = C.c_int64(0)
+    rc = lib().qvo_velocity(C.byref(params), n, _d(x), _i(inf), _i(beh), _d(u), _d(uli), _d(ubs),
+                            C.byref(ev), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"qvo_velocity failed ({rc})")
+    return u, uli, ubs, int(ev.value)
+
+
+def velocity_targets(params: QvoParams, fil, targets, nthreads: int = 0):
+    x, inf, beh = _prep(fil)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    u = np.zeros((t.shape[0], 3))
+    ev = C.c_int64(0)
+    bs, ws = C.c_double(0), C.c_double(0)
+    rc = lib().qvo_velocity_targets(C.byref(params), x.shape[0], _d(x), _i(inf), _i(beh), _i(t),
+                                    t.shape[0], _d(u), C.byref(ev), nthreads, C.byref(bs),
+                                    C.byref(ws))
+    if rc != 0:
+        raise RuntimeError(f"qvo_velocity_targets failed ({rc})")
+    return u, int(ev.value), bs.value, ws.value
+
+
+def tree_dump(params: QvoParams, fil) -> np.ndarray:
+    x, inf, beh = _prep(fil)
+    n = lib().qvo_tree_dump(C.byref(params), x.shape[0], _d(x), _i(inf), _i(beh), None, 0)
+    if n < 0:
+        raise RuntimeError("qvo_tree_dump failed")
+    out = np.zeros((n, NODE_DOUBLES))
+    lib().qvo_tree_dump(C.byref(params), x.shape[0], _d(x), _i(inf), _i(beh), _d(out), n)
+    return out
+
+
+def ghostp(params: QvoParams, fil):
+    x, inf, beh = _prep(fil)
+    n = x.shape[0]
+    g = [np.zeros((n, 3)) for _ in range(4)]
+    lib().qvo_ghostp(C.byref(params), n, _d(x), _i(inf), _i(beh), *[_d(a) for a in g])
+    return g
+
+
+def ab_update(params: QvoParams, x, infront, u, u1, u2, u3):
+    """In-place AB ladder on copies; returns (x, u1, u2, u3)."""
+    x = np.array(x, dtype=np.float64, order="C"); u1 = np.array(u1, dtype=np.float64, order="C")
+    u2 = np.array(u2, dtype=np.float64, order="C"); u3 = np.array(u3, dtype=np.float64, order="C")
+    inf = np.ascontiguousarray(infront, dtype=np.int32)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    lib().qvo_ab_update(C.byref(params), x.shape[0], _d(x), _i(inf), _d(uu), _d(u1), _d(u2), _d(u3))
+    return x, u1, u2, u3
+
+
+def run(params: QvoParams, fil, nsteps: int, u1=None, u2=None, u3=None, nthreads: int = 0):
+    x, inf, beh = _prep(fil)
+    x = x.copy()
+    n = x.shape[0]
+    u1 = np.zeros((n, 3)) if u1 is None else np.array(u1, dtype=np.float64, order="C")
+    u2 = np.zeros((n, 3)) if u2 is None else np.array(u2, dtype=np.float64, order="C")
+    u3 = np.zeros((n, 3)) if u3 is None else np.array(u3, dtype=np.float64, order="C")
+    uf = np.zeros((n, 3))
+    rc = lib().qvo_run(C.byref(params), n, _d(x), _i(inf), _i(beh), _d(u1), _d(u2), _d(u3),
+                       nsteps, _d(uf), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"qvo_run failed ({rc})")
+    return x, u1, u2, u3, uf

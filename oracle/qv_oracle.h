@@ -1,0 +1,87 @@
+/*
+ * qv_oracle.h -- CPU restatement of Qvort's nonlocal filament velocity.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker or
+ * the timed CPU baseline.  The product (libqvort_gpu.so) never links it.
+ *
+ * Conventions follow the reference's data model (cdata.f90:20-41):
+ *   x[3*i+k]    position of slot i (0-based slot, k = axis), FP64
+ *   infront[i], behind[i]  1-based slot links, 0 marks an empty slot
+ */
+#ifndef QV_ORACLE_H
+#define QV_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QVO_VEL_BS = 1, QVO_VEL_TREE = 2 };
+enum { QVO_BC_OPEN = 0, QVO_BC_PERIODIC = 1, QVO_BC_OPENX = 2, QVO_BC_OPENXY = 3 };
+
+typedef struct qvo_params {
+  double quant_circ;   /* Gamma        cdata.f90:202 */
+  double corea;        /* a            cdata.f90:203 */
+  double box_size;     /* L            cdata.f90:200 */
+  double delta;        /* delta        cdata.f90:198 */
+  double tree_theta;   /* theta        cdata.f90:331 */
+  double dt;           /* dt           cdata.f90:199 */
+  int32_t velocity;    /* QVO_VEL_*    cdata.f90:210 */
+  int32_t boundary;    /* QVO_BC_*     initial.f90:104-136 */
+  int32_t tree_extra_correction; /*    cdata.f90:333 */
+  int32_t deriv_order; /* 2 or 4       cdata.f90:218 */
+} qvo_params;
+
+/* node record written by qvo_tree_dump: 22 doubles per node, DFS order,
+ * empty children included (the order of tree_walk's recursion) */
+enum { QVO_NODE_DOUBLES = 22 };
+
+int qvo_ghostp(const qvo_params *p, int32_t pcount, const double *x,
+               const int32_t *infront, const int32_t *behind,
+               double *gi, double *gb, double *gii, double *gbb);
+
+/* full velocity of every active slot (calc_velocity, timestep.f90:180-309);
+ * outputs are zero for empty slots.  evals: eval_counter after the stage
+ * (Tree mode, 64-bit).  nthreads<=0: OpenMP default. */
+int qvo_velocity(const qvo_params *p, int32_t pcount, const double *x,
+                 const int32_t *infront, const int32_t *behind,
+                 double *u, double *u_li, double *u_bs, int64_t *evals,
+                 int32_t nthreads);
+
+/* velocity of the listed 0-based targets only (bounded CPU baseline);
+ * writes u[3*k] for target k; returns 0 or a negative error.
+ * build_s / walk_s receive the wall times of the tree build and the walk. */
+int qvo_velocity_targets(const qvo_params *p, int32_t pcount, const double *x,
+                         const int32_t *infront, const int32_t *behind,
+                         const int32_t *targets, int32_t ntargets, double *u,
+                         int64_t *evals, int32_t nthreads,
+                         double *build_s, double *walk_s);
+
+/* nodes of the octree (construct_tree, tree.f90:44-186) in DFS order.
+ * Returns the node count; writes at most max_nodes records into out. */
+int64_t qvo_tree_dump(const qvo_params *p, int32_t pcount, const double *x,
+                      const int32_t *infront, const int32_t *behind,
+                      double *out, int64_t max_nodes);
+
+/* Adams-Bashforth ladder of pmotion (timestep.f90:48-71), in place */
+int qvo_ab_update(const qvo_params *p, int32_t pcount, double *x,
+                  const int32_t *infront, const double *u,
+                  double *u1, double *u2, double *u3);
+
+/* periodic wrap after a step (periodic.f90:188-276) */
+int qvo_enforce_periodic(const qvo_params *p, int32_t pcount, double *x,
+                         const int32_t *infront);
+
+/* nsteps of the reference main loop without line operations
+ * (run.f90:41-54, 75-107): ghostp, tree, velocity, AB, ghostp, wrap.
+ * x/u1/u2/u3 updated in place; u_first receives the first step's velocity. */
+int qvo_run(const qvo_params *p, int32_t pcount, double *x,
+            const int32_t *infront, const int32_t *behind,
+            double *u1, double *u2, double *u3, int32_t nsteps,
+            double *u_first, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

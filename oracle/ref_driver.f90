@@ -1,0 +1,206 @@
+!> Test-infrastructure driver around the reference Qvort hot path.
+!>
+!> This program is NOT product code.  It is linked (by oracle/Makefile) against
+!> objects compiled from the reference sources under /root/reference/src and is
+!> used only (a) to write golden vectors under tests/golden/ and (b) as the
+!> "reference" CPU baseline timed by bench.py.  It calls the reference's own
+!> routines unchanged:
+!>   ghostp          periodic.f90:8-17
+!>   construct_tree  tree.f90:44-80
+!>   pmotion         timestep.f90:19-93 (calc_velocity timestep.f90:180-370)
+!>   enforce_periodic* periodic.f90:188-303
+!>   empty_tree      tree.f90:189-203
+!> and mirrors the order of the main loop in run.f90:36-169 with the line
+!> operations (pinsert/precon/premove) left out.
+!>
+!> Files (stream access, little endian), in the working directory:
+!>   run.in     read by the reference's read_run_file (cdata.f90:420-807)
+!>   state.bin  int32 version, pcount, nsteps, flags, nsample
+!>              real64 x(3,pcount); int32 infront(pcount), behind(pcount)
+!>              real64 u1(3,pcount), u2(3,pcount), u3(3,pcount)
+!>   out.bin    int32 pcount; int64 eval_counter (first step)
+!>              real64 u, u_s_LI, u_s_BS, ghosti, ghostb   (3,pcount each, first step)
+!>              real64 x, u1, u2, u3                        (3,pcount each, after nsteps)
+!>   tree.bin   (flags bit 0) every node of the first step's tree in DFS order
+!>   timing.txt (flags bit 1) build/walk wall times of a target sample
+program qv_ref_driver
+  use cdata
+  use general
+  use periodic
+  use tree
+  use timestep
+  use omp_lib
+  implicit none
+  integer :: version, nsteps_run, flags, nsample
+  integer :: i, k, stride, nact, nwalk
+  integer(8) :: eval_first
+  real, allocatable :: u_first(:,:), uli_first(:,:), ubs_first(:,:), gi_first(:,:), gb_first(:,:)
+  real :: uvel(3)
+  real(8) :: t0, t1, t2
+  integer :: nodes_written
+
+  call read_run_file()
+  call set_boundary_flags()
+
+  open(unit=41, file='state.bin', access='stream', form='unformatted', status='old', action='read')
+  read(41) version, pcount, nsteps_run, flags, nsample
+  allocate(f(pcount))
+  do i = 1, pcount
+    f(i)%x = 0.; f(i)%u = 0.; f(i)%u1 = 0.; f(i)%u2 = 0.; f(i)%u3 = 0.
+    f(i)%ghosti = 0.; f(i)%ghostb = 0.; f(i)%ghostii = 0.; f(i)%ghostbb = 0.
+    f(i)%u_s_LI = 0.; f(i)%u_s_BS = 0.; f(i)%u_sup = 0.; f(i)%u_mf = 0.
+    f(i)%closest = 0; f(i)%closestd = 0.
+  end do
+  do i = 1, pcount
+    read(41) f(i)%x
+  end do
+  do i = 1, pcount
+    read(41) f(i)%infront
+  end do
+  do i = 1, pcount
+    read(41) f(i)%behind
+  end do
+  do i = 1, pcount
+    read(41) f(i)%u1
+  end do
+  do i = 1, pcount
+    read(41) f(i)%u2
+  end do
+  do i = 1, pcount
+    read(41) f(i)%u3
+  end do
+  close(41)
+
+  if (btest(flags, 1)) then
+    call timing_run()
+    stop
+  end if
+
+  allocate(u_first(3,pcount), uli_first(3,pcount), ubs_first(3,pcount))
+  allocate(gi_first(3,pcount), gb_first(3,pcount))
+  eval_first = 0
+  t = 0.
+  do itime = 1, nsteps_run
+    call ghostp
+    if (tree_theta > 0) call construct_tree
+    if (itime == 1 .and. btest(flags, 0) .and. tree_theta > 0) then
+      open(unit=42, file='tree.bin', access='stream', form='unformatted', status='replace')
+      nodes_written = 0
+      call dump_node(vtree, 0)
+      close(42)
+    end if
+    call pmotion
+    if (itime == 1) then
+      do i = 1, pcount
+        u_first(:,i) = f(i)%u
+        uli_first(:,i) = f(i)%u_s_LI
+        ubs_first(:,i) = f(i)%u_s_BS
+        gi_first(:,i) = f(i)%ghosti
+        gb_first(:,i) = f(i)%ghostb
+      end do
+      if (tree_theta > 0) eval_first = eval_counter
+    end if
+    call ghostp
+    select case(boundary)
+      case('periodic')
+        call enforce_periodic
+      case('openx')
+        call enforce_periodic_yz
+      case('openxy')
+        call enforce_periodic_z
+    end select
+    t = t + dt
+    if (tree_theta > 0) then
+      call empty_tree(vtree)
+      deallocate(vtree%parray) ; deallocate(vtree)
+      nullify(vtree)
+    end if
+  end do
+
+  open(unit=43, file='out.bin', access='stream', form='unformatted', status='replace')
+  write(43) pcount
+  write(43) eval_first
+  write(43) u_first, uli_first, ubs_first, gi_first, gb_first
+  do i = 1, pcount
+    write(43) f(i)%x
+  end do
+  do i = 1, pcount
+    write(43) f(i)%u1
+  end do
+  do i = 1, pcount
+    write(43) f(i)%u2
+  end do
+  do i = 1, pcount
+    write(43) f(i)%u3
+  end do
+  close(43)
+
+contains
+
+  !> boundary -> periodic flags, as init_setup does (initial.f90:104-136)
+  subroutine set_boundary_flags()
+    select case(boundary)
+      case('periodic')
+        periodic_bc = .true.
+      case('openx')
+        periodic_bc_notx = .true.
+      case('openxy')
+        periodic_bc_notxy = .true.
+    end select
+  end subroutine
+
+  !> every node (empty children included) in the DFS order of tree_walk
+  recursive subroutine dump_node(nd, level)
+    type(node), pointer :: nd
+    integer, intent(in) :: level
+    integer :: leafj
+    leafj = 0
+    if (nd%pcount == 1) then
+      if (nd%parray(1)%infront > 0) leafj = f(nd%parray(1)%infront)%behind
+    end if
+    write(42) level, nd%pcount, leafj
+    write(42) nd%posx, nd%posy, nd%posz, nd%width
+    write(42) nd%centx, nd%centy, nd%centz
+    write(42) nd%circ
+    write(42) nd%Cij
+    nodes_written = nodes_written + 1
+    if (nd%pcount > 1) then
+      call dump_node(nd%fbl, level+1) ; call dump_node(nd%fbr, level+1)
+      call dump_node(nd%ftl, level+1) ; call dump_node(nd%ftr, level+1)
+      call dump_node(nd%bbl, level+1) ; call dump_node(nd%bbr, level+1)
+      call dump_node(nd%btl, level+1) ; call dump_node(nd%btr, level+1)
+    end if
+  end subroutine
+
+  !> time construct_tree and calc_velocity over a strided sample of targets
+  subroutine timing_run()
+    integer :: ii
+    itime = 1
+    t = 0.
+    nact = count(mask=f(:)%infront > 0)
+    if (nsample <= 0 .or. nsample > nact) nsample = nact
+    stride = max(1, nact / nsample)
+    call ghostp
+    t0 = omp_get_wtime()
+    if (tree_theta > 0) call construct_tree
+    t1 = omp_get_wtime()
+    nwalk = 0
+    !$omp parallel do private(ii, uvel) reduction(+:nwalk) schedule(dynamic, 16)
+    do ii = 1, pcount, stride
+      if (f(ii)%infront == 0) cycle
+      call calc_velocity(uvel, ii)
+      f(ii)%u = uvel
+      nwalk = nwalk + 1
+    end do
+    !$omp end parallel do
+    t2 = omp_get_wtime()
+    open(unit=44, file='timing.txt', status='replace')
+    write(44, '(a,i0)') 'n_active ', nact
+    write(44, '(a,i0)') 'n_sample ', nwalk
+    write(44, '(a,i0)') 'threads ', omp_get_max_threads()
+    write(44, '(a,es24.16)') 'build_s ', t1 - t0
+    write(44, '(a,es24.16)') 'walk_s ', t2 - t1
+    close(44)
+    k = 0
+  end subroutine
+end program

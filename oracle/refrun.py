@@ -1,0 +1,100 @@
+"""Run the flang build of the reference hot path (oracle/_ref/qvort_ref_driver).
+
+Test infrastructure only (see oracle/__init__.py).  The driver is built by
+``make -C oracle ref`` from the reference sources; on a machine without them
+the prebuilt driver is used if present.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "_ref", "qvort_ref_driver")
+NODE_BYTES = 3 * 4 + 4 * 8 + 3 * 8 + 3 * 8 + 9 * 8
+
+
+def available() -> bool:
+    return os.path.exists(DRIVER)
+
+
+def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.1,
+                delta=5e-4, dt=1e-6, quant_circ=9.97e-4, corea=8.244023e-9,
+                tree_extra_correction=True, deriv_order="second", nsteps=1) -> str:
+    lines = [
+        f"nsteps {nsteps}", "shots 1000000", "recon_shots 1000000", f"dt {dt!r}",
+        f"delta {delta!r}", f"box_size {box_size!r}", f"velocity {velocity}",
+        f"boundary {boundary}", f"quant_circ {quant_circ!r}", f"corea {corea!r}",
+        f"tree_theta {tree_theta!r}",
+        f"tree_extra_correction {'T' if tree_extra_correction else 'F'}",
+        f"deriv_order {deriv_order}", "initf single_loop", "pcount 16",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def _write_state(path, fil, nsteps, flags, nsample, u1=None, u2=None, u3=None):
+    n = fil.pcount
+    z = np.zeros((n, 3))
+    with open(path, "wb") as fh:
+        np.array([1, n, nsteps, flags, nsample], dtype="<i4").tofile(fh)
+        np.ascontiguousarray(fil.x, dtype="<f8").tofile(fh)
+        np.ascontiguousarray(fil.infront, dtype="<i4").tofile(fh)
+        np.ascontiguousarray(fil.behind, dtype="<i4").tofile(fh)
+        for a in (u1, u2, u3):
+            np.ascontiguousarray(z if a is None else a, dtype="<f8").tofile(fh)
+
+
+def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None,
+        u1=None, u2=None, u3=None, timeout=3600, **params) -> dict:
+    """Run the reference driver; returns a dict of numpy arrays."""
+    if not available():
+        raise FileNotFoundError(DRIVER)
+    flags = (1 if dump_tree else 0) | (2 if timing else 0)
+    env = dict(os.environ)
+    if threads is not None:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory(prefix="qvref_") as wd:
+        with open(os.path.join(wd, "run.in"), "w") as fh:
+            fh.write(run_in_text(nsteps=nsteps, **params))
+        os.makedirs(os.path.join(wd, "data"), exist_ok=True)
+        _write_state(os.path.join(wd, "state.bin"), fil, nsteps, flags, nsample, u1, u2, u3)
+        proc = subprocess.run([DRIVER], cwd=wd, env=env, capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0 or "FATAL" in proc.stdout:
+            raise RuntimeError(f"reference driver failed: {proc.stdout[-2000:]}{proc.stderr[-2000:]}")
+        out = {"stdout": proc.stdout}
+        if timing:
+            with open(os.path.join(wd, "timing.txt")) as fh:
+                for line in fh:
+                    k, v = line.split()
+                    out[k] = float(v)
+            return out
+        n = fil.pcount
+        raw = open(os.path.join(wd, "out.bin"), "rb").read()
+        pc = np.frombuffer(raw, dtype="<i4", count=1)[0]
+        assert pc == n
+        out["evals"] = int(np.frombuffer(raw, dtype="<i8", count=1, offset=4)[0])
+        arr = np.frombuffer(raw, dtype="<f8", offset=12).reshape(9, n, 3)
+        for k, name in enumerate(["u", "u_li", "u_bs", "ghosti", "ghostb", "x", "u1", "u2", "u3"]):
+            out[name] = arr[k].copy()
+        if dump_tree:
+            out["tree"] = parse_tree(open(os.path.join(wd, "tree.bin"), "rb").read())
+        return out
+
+
+def parse_tree(raw: bytes) -> np.ndarray:
+    """Node records as rows of 22 doubles, the layout of the C oracle's dump:
+    level, pcount, leafj, pos(3), width, cent(3), circ(3), Cij(9) column-major."""
+    dt = np.dtype([("i", "<i4", 3), ("pos", "<f8", 4), ("cent", "<f8", 3), ("circ", "<f8", 3),
+                   ("C", "<f8", 9)])
+    rec = np.frombuffer(raw, dtype=dt)
+    out = np.zeros((rec.shape[0], 22))
+    out[:, 0:3] = rec["i"]
+    out[:, 3:7] = rec["pos"]
+    out[:, 7:10] = rec["cent"]
+    out[:, 10:13] = rec["circ"]
+    out[:, 13:22] = rec["C"]
+    return out
