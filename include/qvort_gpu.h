@@ -1,0 +1,131 @@
+/*
+ * qvort_gpu.h -- C ABI of the MI355X (gfx950) filament-velocity library.
+ *
+ * Drop-in for Qvort's per-timestep velocity hot path.  The reference has no
+ * FFI: the hot path is a set of Fortran module procedures acting on global
+ * state (cdata.f90:20-41).  Each entry point below replaces one of them and
+ * is what a Fortran `bind(C)` interface (qvort_amd/fortran/qv_gpu.f90) binds:
+ *
+ *   qv_upload            the f(:)%x / infront / behind arrays the hot path
+ *                        reads (cdata.f90:20-41), 1-based links, 0 = empty
+ *   qv_velocity          ghostp (periodic.f90:8-17) + construct_tree
+ *                        (tree.f90:44-80) + the velocity loop of pmotion
+ *                        (timestep.f90:41-47 -> calc_velocity :180-370,
+ *                        tree_walk tree.f90:336-421, biot_savart
+ *                        timestep.f90:462-507)
+ *   qv_ab_step           the Adams-Bashforth ladder of pmotion
+ *                        (timestep.f90:48-71)
+ *   qv_enforce_periodic  enforce_periodic (periodic.f90:188-213)
+ *   qv_tree_dump         the tree as construct_tree builds it, node by node
+ *                        in tree_walk's DFS order (tree.f90:85-125, 416-419)
+ *   qv_destroy           empty_tree + deallocate(vtree) (run.f90:132-138)
+ *
+ * Conventions:
+ *   - arrays are host pointers in the Fortran layout x(3,pcount): x[3*i+k]
+ *   - links are 1-based; infront[i]==0 marks an empty slot (holes allowed)
+ *   - every call is synchronous on return, ctx is not thread-safe; call it
+ *     from the master thread, outside OpenMP regions
+ *   - every entry point returns a qv_status; nothing calls exit()
+ *   - arithmetic: QV_FP64 reproduces the reference FP64 velocity (decisions
+ *     of the tree walk bit-identical, values within 1e-12 relative);
+ *     QV_FP32C evaluates leaf segments and cell multipoles in FP32 with
+ *     compensated per-target sums and keeps the LIA term, the walk decisions
+ *     and the AB update in FP64.
+ */
+#ifndef QVORT_GPU_H
+#define QVORT_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qv_ctx qv_ctx;
+
+/* run.in `velocity` (cdata.f90:489-491) */
+enum qv_velocity { QV_VEL_BS = 1, QV_VEL_TREE = 2 };
+/* run.in `boundary` -> periodic_bc / _notx / _notxy (initial.f90:104-136) */
+enum qv_boundary { QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, QV_BC_OPENX = 2, QV_BC_OPENXY = 3 };
+/* arithmetic of the nonlocal sum */
+enum qv_precision { QV_FP64 = 0, QV_FP32C = 1 };
+
+enum qv_status {
+  QV_OK = 0,
+  QV_ERR_ARG = 1,        /* invalid argument or parameter combination */
+  QV_ERR_HIP = 2,        /* HIP runtime error (device, allocation, launch) */
+  QV_ERR_SINGULAR = 3,   /* target coincides with a leaf: the fatal_error of
+                            tree.f90:372-376 ("singularity in BS (tree)
+                            velocity field") */
+  QV_ERR_TREE_DEPTH = 4, /* two points closer than box_size/2^32: the octree
+                            would need more than 32 levels */
+  QV_ERR_STATE = 5,      /* call out of order (e.g. qv_ab_step before any
+                            qv_velocity) */
+  QV_ERR_NO_DEVICE = 6   /* no HIP device visible */
+};
+
+/* the run.in parameters of the hot path (cdata.f90:196-416) */
+typedef struct qv_params {
+  double quant_circ;             /* Gamma, `quant_circ` */
+  double corea;                  /* a, `corea` */
+  double box_size;               /* L, `box_size` (> 0 for every boundary) */
+  double delta;                  /* `delta` (not used by the velocity) */
+  double tree_theta;             /* `tree_theta` (Tree: > 0) */
+  double dt;                     /* `dt` */
+  int32_t velocity;              /* enum qv_velocity */
+  int32_t boundary;              /* enum qv_boundary */
+  int32_t tree_extra_correction; /* `tree_extra_correction` (reference default 1) */
+  int32_t deriv_order;           /* `deriv_order`: 2 ('second') or 4 ('fourth') */
+  int32_t precision;             /* enum qv_precision */
+  int32_t reserved;              /* must be 0 */
+} qv_params;
+
+/* per-stage timings of the last qv_velocity (device events, milliseconds) */
+typedef struct qv_timings {
+  float ghost_lia_ms;  /* ghost gather + LIA */
+  float build_ms;      /* keys + sort + nodes + moments (Tree only) */
+  float walk_ms;       /* tree walk or direct sum, incl. final composition */
+  float total_ms;      /* whole qv_velocity on the device */
+  int64_t nodes;       /* non-empty tree nodes (Tree only) */
+  int64_t evals;       /* interactions (reference eval_counter, 64-bit) */
+} qv_timings;
+
+int qv_create(const qv_params *params, qv_ctx **out);
+int qv_destroy(qv_ctx *ctx);
+int qv_set_params(qv_ctx *ctx, const qv_params *params);
+
+/* copy f(:)%x, infront, behind to the device; resets the AB history to 0 */
+int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x,
+              const int32_t *infront, const int32_t *behind);
+/* AB history f(:)%u1..u3 (NULL = leave as is) */
+int qv_upload_history(qv_ctx *ctx, const double *u1, const double *u2,
+                      const double *u3);
+
+/* velocity of every slot: u = f(:)%u, u_s_li = f(:)%u_s_LI, u_s_bs =
+ * f(:)%u_s_BS (any output may be NULL; empty slots get 0).  evals (may be
+ * NULL): eval_counter of the stage (Tree), 64-bit. */
+int qv_velocity(qv_ctx *ctx, double *u, double *u_s_li, double *u_s_bs,
+                int64_t *evals);
+/* AB ladder + history shift with the velocity of the last qv_velocity */
+int qv_ab_step(qv_ctx *ctx);
+/* periodic wrap of the device positions */
+int qv_enforce_periodic(qv_ctx *ctx);
+/* nsteps of: velocity, AB step, wrap -- all on the device */
+int qv_step(qv_ctx *ctx, int32_t nsteps);
+/* device state back to the host (NULL = skip) */
+int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2,
+                double *u3);
+
+/* the tree of the last Tree-mode qv_velocity, every node (empty children
+ * included) in DFS order, 22 doubles per node: level, pcount, leaf index j
+ * (1-based, 0 if none), posx, posy, posz, width, cent(3), circ(3), Cij(9)
+ * column-major.  nnodes receives the node count; at most max_nodes records
+ * are written (out may be NULL to query the count). */
+int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes);
+
+int qv_get_timings(qv_ctx *ctx, qv_timings *t);
+const char *qv_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QVORT_GPU_H */

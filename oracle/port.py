@@ -61,6 +61,10 @@ def lib():
         L.qvo_ab_update.restype = C.c_int
         L.qvo_run.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, C.c_int32, d, C.c_int32]
         L.qvo_run.restype = C.c_int
+        L.qvo_segment_velocity.argtypes = [P(QvoParams), d, d, d, d, C.c_int32, d]
+        L.qvo_segment_velocity.restype = C.c_int
+        L.qvo_enforce_periodic.argtypes = [P(QvoParams), C.c_int32, d, i32]
+        L.qvo_enforce_periodic.restype = C.c_int
         _lib = L
     return _lib
 
@@ -148,3 +152,11 @@ def run(params: QvoParams, fil, nsteps: int, u1=None, u2=None, u3=None, nthreads
     if rc != 0:
         raise RuntimeError(f"qvo_run failed ({rc})")
     return x, u1, u2, u3, uf
+
+
+def segment_velocity(params: QvoParams, xi, xj, gj, shift=(0.0, 0.0, 0.0), form: int = 0):
+    """Velocity at xi of the straight segment xj -> gj shifted by `shift`."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (xi, xj, gj, shift)]
+    u = np.zeros(3)
+    lib().qvo_segment_velocity(C.byref(params), *[_d(v) for v in a], form, _d(u))
+    return u

@@ -689,15 +689,50 @@ int qvo_ab_update(const qvo_params *p, int32_t pcount, double *x,
 
 int qvo_enforce_periodic(const qvo_params *p, int32_t pcount, double *x,
                          const int32_t *infront) {
-  if (p->boundary != QVO_BC_PERIODIC) return 0;
+  /* wrap part of enforce_periodic / _yz / _z; the loop removal of the open
+   * axes (boundary_loop_remove) is host topology and not restated here */
+  int k0 = 3;
+  if (p->boundary == QVO_BC_PERIODIC) k0 = 0;
+  else if (p->boundary == QVO_BC_OPENX) k0 = 1;
+  else if (p->boundary == QVO_BC_OPENXY) k0 = 2;
   const double L = p->box_size;
   for (int i = 0; i < pcount; ++i) {
     if (infront[i] == 0) continue;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = k0; k < 3; ++k) {
       if (x[3 * i + k] > (L / 2.)) x[3 * i + k] = x[3 * i + k] - L;
       else if (x[3 * i + k] < (-L / 2.)) x[3 * i + k] = x[3 * i + k] + L;
     }
   }
+  return 0;
+}
+
+/* one segment's velocity at a target: form 0 = biot_savart_shift
+ * (timestep.f90:497-505, without the Gamma/(2 pi D) prefactor folded in
+ * differently), form 1 = tree_walk's leaf (tree.f90:380-392); both include
+ * Gamma.  For unit tests of the pair formulas. */
+int qvo_segment_velocity(const qvo_params *p, const double *xi, const double *xj,
+                         const double *gj, const double *shift, int32_t form,
+                         double *u) {
+  u[0] = u[1] = u[2] = 0.;
+  if (form == 0) {
+    double xs[6] = {xi[0], xi[1], xi[2], xj[0], xj[1], xj[2]};
+    double gs[6] = {0., 0., 0., gj[0], gj[1], gj[2]};
+    int32_t inf[2] = {2, 1}, beh[2] = {2, 1};
+    qvo_state s = {p, 2, xs, inf, beh, gs, gs, gs, gs};
+    s.behind = beh;
+    bs_pair(&s, 0, 1, shift, u);
+    return 0;
+  }
+  double R1[3], R2[3], ub[3];
+  for (int k = 0; k < 3; ++k) {
+    R1[k] = (xj[k] + shift[k]) - xi[k];
+    R2[k] = (gj[k] + shift[k]) - xi[k];
+  }
+  const double a = vnorm(R1), b = vnorm(R2);
+  const double c = (a * b) * ((a * b) + dot3(R1, R2));
+  cross(R1, R2, ub);
+  const double f = (a + b) / c;
+  for (int k = 0; k < 3; ++k) u[k] = ((f * ub[k] * p->quant_circ) / 4.) / QVO_PI;
   return 0;
 }
 

@@ -69,9 +69,15 @@ int qvo_ab_update(const qvo_params *p, int32_t pcount, double *x,
                   const int32_t *infront, const double *u,
                   double *u1, double *u2, double *u3);
 
-/* periodic wrap after a step (periodic.f90:188-276) */
+/* wrap after a step: enforce_periodic / _yz / _z (periodic.f90:188-276) */
 int qvo_enforce_periodic(const qvo_params *p, int32_t pcount, double *x,
                          const int32_t *infront);
+
+/* one straight segment x_j -> g_j (+shift) seen from x_i; form 0 is the
+ * biot_savart pair formula, form 1 the tree leaf formula */
+int qvo_segment_velocity(const qvo_params *p, const double *xi, const double *xj,
+                         const double *gj, const double *shift, int32_t form,
+                         double *u);
 
 /* nsteps of the reference main loop without line operations
  * (run.f90:41-54, 75-107): ghostp, tree, velocity, AB, ghostp, wrap.

@@ -1,0 +1,427 @@
+// qv_api.hip -- C ABI of libqvort_gpu.so: context, device buffers, the
+// velocity stage, the AB step and the tree dump.  Declarations and the
+// reference routine each entry point replaces: include/qvort_gpu.h.
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "qv_internal.h"
+
+namespace qv {
+namespace {
+
+int check_params(const qv_params *p) {
+  if (!p) return QV_ERR_ARG;
+  if (p->velocity != QV_VEL_BS && p->velocity != QV_VEL_TREE) return QV_ERR_ARG;
+  if (p->boundary < QV_BC_OPEN || p->boundary > QV_BC_OPENXY) return QV_ERR_ARG;
+  if (!(p->box_size > 0.)) return QV_ERR_ARG;  // initial.f90:155-156
+  if (p->velocity == QV_VEL_TREE && !(p->tree_theta > 0.)) return QV_ERR_ARG;  // initial.f90:356-359
+  if (p->deriv_order != 2 && p->deriv_order != 4) return QV_ERR_ARG;
+  if (p->precision != QV_FP64 && p->precision != QV_FP32C) return QV_ERR_ARG;
+  if (!(p->quant_circ > 0.) || !(p->corea > 0.)) return QV_ERR_ARG;
+  if (p->reserved != 0) return QV_ERR_ARG;
+  return QV_OK;
+}
+
+template <class T>
+hipError_t dalloc(T *&p, size_t n) {
+  (void)hipFree(p);
+  p = nullptr;
+  return hipMalloc(&p, (n ? n : 1) * sizeof(T));
+}
+
+hipError_t valloc(Vec3 &v, size_t n) {
+  hipError_t e;
+  if ((e = dalloc(v.x, n))) return e;
+  if ((e = dalloc(v.y, n))) return e;
+  return dalloc(v.z, n);
+}
+
+void vfree(Vec3 &v) {
+  (void)hipFree(v.x); (void)hipFree(v.y); (void)hipFree(v.z);
+  v = Vec3{};
+}
+
+hipError_t vzero(Vec3 &v, size_t n, hipStream_t s) {
+  hipError_t e;
+  if ((e = hipMemsetAsync(v.x, 0, n * sizeof(double), s))) return e;
+  if ((e = hipMemsetAsync(v.y, 0, n * sizeof(double), s))) return e;
+  return hipMemsetAsync(v.z, 0, n * sizeof(double), s);
+}
+
+#define TRY(expr)                              \
+  do {                                         \
+    hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) return QV_ERR_HIP;   \
+  } while (0)
+
+int reserve(Ctx &c, int32_t n) {
+  if (n <= c.cap) return QV_OK;
+  const size_t cap = (size_t)n;
+  Vec3 *vs[] = {&c.x, &c.gi, &c.gb, &c.gii, &c.gbb, &c.u, &c.uli, &c.ubs,
+                &c.u1, &c.u2, &c.u3, &c.xs, &c.gs};
+  for (Vec3 *v : vs) TRY(valloc(*v, cap));
+  TRY(dalloc(c.infront, cap));
+  TRY(dalloc(c.behind, cap));
+  TRY(dalloc(c.stage, 3 * cap));
+  TRY(dalloc(c.key0, cap));
+  TRY(dalloc(c.key1, 2 * cap));
+  TRY(dalloc(c.key0_s, cap));
+  TRY(dalloc(c.slot, cap));
+  TRY(dalloc(c.slot_s, cap));
+  TRY(dalloc(c.nnode_of, cap + 1));
+  TRY(dalloc(c.node_off, cap + 1));
+  c.cap = n;
+  return QV_OK;
+}
+
+// device (pcount,3) SoA -> host AoS through the staging buffer
+int download_vec(Ctx &c, Vec3 v, double *host) {
+  if (!host) return QV_OK;
+  TRY(launch_soa_to_aos(c, v, c.stage));
+  TRY(hipMemcpyAsync(host, c.stage, (size_t)3 * c.pcount * sizeof(double),
+                     hipMemcpyDeviceToHost, c.stream));
+  return QV_OK;
+}
+
+int upload_vec(Ctx &c, const double *host, Vec3 v) {
+  if (!host) return QV_OK;
+  TRY(hipMemcpyAsync(c.stage, host, (size_t)3 * c.pcount * sizeof(double),
+                     hipMemcpyHostToDevice, c.stream));
+  TRY(launch_aos_to_soa(c, c.stage, v));
+  return QV_OK;
+}
+
+int velocity_stage(Ctx &c) {
+  if (c.pcount == 0) return QV_ERR_STATE;
+  TRY(hipEventRecord(c.ev[0], c.stream));
+  TRY(hipMemsetAsync(c.counters, 0, 4 * sizeof(int32_t), c.stream));
+  TRY(hipMemsetAsync(c.evals, 0, sizeof(int64_t), c.stream));
+  TRY(launch_ghost_lia(c));
+  TRY(hipEventRecord(c.ev[1], c.stream));
+  int rc = QV_OK;
+  if (c.p.velocity == QV_VEL_TREE) {
+    if (c.nact > 0) {
+      rc = tree_build(c);
+      if (rc != QV_OK) return rc;
+    }
+    TRY(hipEventRecord(c.ev[2], c.stream));
+    if (c.nact > 0) TRY(launch_walk(c));
+  } else {
+    TRY(hipEventRecord(c.ev[2], c.stream));
+    TRY(launch_direct(c));
+  }
+  TRY(hipEventRecord(c.ev[3], c.stream));
+  TRY(hipMemcpyAsync(c.h_counters, c.counters, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                     c.stream));
+  TRY(hipMemcpyAsync(c.h_evals, c.evals, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  TRY(hipStreamSynchronize(c.stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
+  c.timings.ghost_lia_ms = ms;
+  (void)hipEventElapsedTime(&ms, c.ev[1], c.ev[2]);
+  c.timings.build_ms = ms;
+  (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]);
+  c.timings.walk_ms = ms;
+  (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[3]);
+  c.timings.total_ms = ms;
+  c.timings.nodes = c.p.velocity == QV_VEL_TREE ? c.tree.nnodes : 0;
+  if (c.p.velocity == QV_VEL_TREE) {
+    // eval_counter starts at 2N (tree.f90:77-78) and counts every leaf and
+    // accepted-cell interaction
+    c.timings.evals = 2 * (int64_t)c.nact + c.h_evals[0];
+  } else {
+    const int64_t S = make_shells(c.p).n;
+    c.timings.evals = S * (int64_t)c.nact * (int64_t)c.nact;
+  }
+  if (c.h_counters[1] & kFlagSingular) return QV_ERR_SINGULAR;
+  c.have_velocity = true;
+  return QV_OK;
+}
+
+struct DumpCtx {
+  const std::vector<int8_t> &level, &digit;
+  const std::vector<int32_t> &count;
+  const std::vector<double> &corner, &mp;
+  const std::vector<double2> &geo;
+  const std::vector<double4> &hot;
+  const std::vector<int2> &meta;
+  double *out;
+  int64_t max_nodes, k;
+};
+
+void dump_emit(DumpCtx &D, int lev, int pc, int leafj, const double *pos, double w,
+               const double *cent, const double *circ, const double *C /* row-major */) {
+  if (D.out && D.k < D.max_nodes) {
+    double *o = D.out + 22 * D.k;
+    o[0] = lev; o[1] = pc; o[2] = leafj;
+    o[3] = pos[0]; o[4] = pos[1]; o[5] = pos[2]; o[6] = w;
+    for (int q = 0; q < 3; ++q) { o[7 + q] = cent[q]; o[10 + q] = circ[q]; }
+    for (int kk = 0; kk < 3; ++kk)
+      for (int m = 0; m < 3; ++m) o[13 + 3 * kk + m] = C[3 * m + kk];
+  }
+  ++D.k;
+}
+
+// children of a split node n are n+1, skip(n+1), ... before skip(n); the
+// recursion depth is at most 33
+void dump_rec(DumpCtx &D, int64_t n) {
+  static const double zero[9] = {0};
+  const double cent[3] = {D.hot[n].x, D.hot[n].y, D.hot[n].z};
+  const double *pos = &D.corner[3 * n];
+  const double w = D.geo[n].x;
+  if (D.meta[n].y >= 0) {
+    // leaf: circ = (0 + ghosti) - x, C = x (g-x) - x (g-x) = 0 (tree.f90:160-178)
+    const double *g = &D.mp[12 * n];
+    const double circ[3] = {g[0] - cent[0], g[1] - cent[1], g[2] - cent[2]};
+    dump_emit(D, D.level[n], D.count[n], D.meta[n].y + 1, pos, w, cent, circ, zero);
+    return;
+  }
+  dump_emit(D, D.level[n], D.count[n], 0, pos, w, cent, &D.mp[12 * n], &D.mp[12 * n + 3]);
+  if (D.count[n] <= 1) return;
+  const int64_t end = D.meta[n].x;
+  int64_t ch = n + 1;
+  for (int d = 0; d < 8; ++d) {
+    if (ch < end && D.digit[ch] == d && D.level[ch] == D.level[n] + 1) {
+      dump_rec(D, ch);
+      ch = D.meta[ch].x;
+    } else {
+      const int b[3] = {d & 1, (d >> 2) & 1, (d >> 1) & 1};
+      double cp[3];
+      for (int a = 0; a < 3; ++a) cp[a] = b[a] ? pos[a] + (w / 2) : pos[a];
+      dump_emit(D, D.level[n] + 1, 0, 0, cp, w / 2., zero, zero, zero);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace qv
+
+using namespace qv;
+
+extern "C" {
+
+int qv_create(const qv_params *params, qv_ctx **out) {
+  if (!out) return QV_ERR_ARG;
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return QV_ERR_NO_DEVICE;
+  Ctx *c = new (std::nothrow) Ctx();
+  if (!c) return QV_ERR_HIP;
+  c->p = *params;
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return QV_ERR_HIP;
+  }
+  for (auto &e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return QV_ERR_HIP;
+    }
+  }
+  if (hipMalloc(&c->counters, 8 * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->evals, 2 * sizeof(int64_t)) != hipSuccess ||
+      hipHostMalloc(&c->h_counters, 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->h_evals, 2 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    qv_destroy(reinterpret_cast<qv_ctx *>(c));
+    return QV_ERR_HIP;
+  }
+  *out = reinterpret_cast<qv_ctx *>(c);
+  return QV_OK;
+}
+
+int qv_destroy(qv_ctx *ctx) {
+  if (!ctx) return QV_OK;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  Vec3 *vs[] = {&c->x, &c->gi, &c->gb, &c->gii, &c->gbb, &c->u, &c->uli, &c->ubs,
+                &c->u1, &c->u2, &c->u3, &c->xs, &c->gs};
+  for (Vec3 *v : vs) vfree(*v);
+  void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
+                c->slot_s, c->nnode_of, c->node_off, c->sort_tmp, c->counters, c->evals,
+                c->tree.hot, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
+                c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner};
+  for (void *p : ps) (void)hipFree(p);
+  if (c->h_counters) (void)hipHostFree(c->h_counters);
+  if (c->h_evals) (void)hipHostFree(c->h_evals);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return QV_OK;
+}
+
+int qv_set_params(qv_ctx *ctx, const qv_params *params) {
+  if (!ctx) return QV_ERR_ARG;
+  int rc = check_params(params);
+  if (rc) return rc;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  c->p = *params;
+  c->have_velocity = false;
+  return QV_OK;
+}
+
+int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infront,
+              const int32_t *behind) {
+  if (!ctx || !x || !infront || !behind || pcount < 2) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  // links must stay inside f(:) and point at occupied slots
+  int32_t nact = 0;
+  for (int32_t i = 0; i < pcount; ++i) {
+    const int32_t a = infront[i], b = behind[i];
+    if (a == 0) continue;
+    if (a < 1 || a > pcount || b < 1 || b > pcount) return QV_ERR_ARG;
+    if (infront[a - 1] == 0 || infront[b - 1] == 0) return QV_ERR_ARG;
+    ++nact;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  int rc = reserve(*c, pcount);
+  if (rc) return rc;
+  c->pcount = pcount;
+  c->nact = nact;
+  c->have_velocity = false;
+  TRY(hipMemcpyAsync(c->infront, infront, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c->stream));
+  TRY(hipMemcpyAsync(c->behind, behind, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c->stream));
+  rc = upload_vec(*c, x, c->x);
+  if (rc) return rc;
+  TRY(vzero(c->u1, pcount, c->stream));
+  TRY(vzero(c->u2, pcount, c->stream));
+  TRY(vzero(c->u3, pcount, c->stream));
+  TRY(vzero(c->u, pcount, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_upload_history(qv_ctx *ctx, const double *u1, const double *u2, const double *u3) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  int rc;
+  if ((rc = upload_vec(*c, u1, c->u1))) return rc;
+  if ((rc = upload_vec(*c, u2, c->u2))) return rc;
+  if ((rc = upload_vec(*c, u3, c->u3))) return rc;
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_velocity(qv_ctx *ctx, double *u, double *u_s_li, double *u_s_bs, int64_t *evals) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  int rc = velocity_stage(*c);
+  if (rc) return rc;
+  if ((rc = download_vec(*c, c->u, u))) return rc;
+  if (u) TRY(hipStreamSynchronize(c->stream));
+  if ((rc = download_vec(*c, c->uli, u_s_li))) return rc;
+  if (u_s_li) TRY(hipStreamSynchronize(c->stream));
+  if ((rc = download_vec(*c, c->ubs, u_s_bs))) return rc;
+  TRY(hipStreamSynchronize(c->stream));
+  if (evals) *evals = c->timings.evals;
+  return QV_OK;
+}
+
+int qv_ab_step(qv_ctx *ctx) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c->have_velocity) return QV_ERR_STATE;
+  TRY(launch_ab_update(*c));
+  TRY(hipStreamSynchronize(c->stream));
+  c->have_velocity = false;
+  return QV_OK;
+}
+
+int qv_enforce_periodic(qv_ctx *ctx) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  TRY(launch_enforce_periodic(*c));
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_step(qv_ctx *ctx, int32_t nsteps) {
+  if (!ctx || nsteps < 0) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  for (int32_t k = 0; k < nsteps; ++k) {
+    int rc = velocity_stage(*c);
+    if (rc) return rc;
+    TRY(launch_ab_update(*c));
+    TRY(launch_enforce_periodic(*c));
+    c->have_velocity = false;
+  }
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, double *u3) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  double *hs[] = {x, u, u1, u2, u3};
+  Vec3 vs[] = {c->x, c->u, c->u1, c->u2, c->u3};
+  for (int k = 0; k < 5; ++k) {
+    if (!hs[k]) continue;
+    int rc = download_vec(*c, vs[k], hs[k]);
+    if (rc) return rc;
+    TRY(hipStreamSynchronize(c->stream));
+  }
+  return QV_OK;
+}
+
+int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->p.velocity != QV_VEL_TREE || c->tree.nnodes == 0) return QV_ERR_STATE;
+  const int64_t nn = c->tree.nnodes;
+  std::vector<int8_t> level(nn), digit(nn);
+  std::vector<int32_t> count(nn);
+  std::vector<double> corner(3 * nn), mp(12 * nn);
+  std::vector<double2> geo(nn);
+  std::vector<double4> hot(nn);
+  std::vector<int2> meta(nn);
+  TRY(hipMemcpy(level.data(), c->tree.level, nn, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(digit.data(), c->tree.digit, nn, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(count.data(), c->tree.count, nn * 4, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(corner.data(), c->tree.corner, nn * 24, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(mp.data(), c->tree.mp, nn * 96, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(geo.data(), c->tree.geo, nn * 16, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(hot.data(), c->tree.hot, nn * 32, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(meta.data(), c->tree.meta, nn * 8, hipMemcpyDeviceToHost));
+  // expand to the reference's node list: every split cell has 8 children,
+  // empty ones included, in DFS order (tree.f90:89-114)
+  DumpCtx D{level, digit, count, corner, mp, geo, hot, meta, out, max_nodes, 0};
+  dump_rec(D, 0);
+  const int64_t k = D.k;
+  if (nnodes) *nnodes = k;
+  return QV_OK;
+}
+
+int qv_get_timings(qv_ctx *ctx, qv_timings *t) {
+  if (!ctx || !t) return QV_ERR_ARG;
+  *t = reinterpret_cast<Ctx *>(ctx)->timings;
+  return QV_OK;
+}
+
+const char *qv_status_string(int status) {
+  switch (status) {
+    case QV_OK: return "ok";
+    case QV_ERR_ARG: return "invalid argument";
+    case QV_ERR_HIP: return "HIP runtime error";
+    case QV_ERR_SINGULAR:
+      return "tree.mod:tree_walk: singularity in BS (tree) velocity field - this is normally "
+             "caused by having recon_shots too large";
+    case QV_ERR_TREE_DEPTH: return "points closer than box_size/2^32: octree deeper than 32 levels";
+    case QV_ERR_STATE: return "call out of order";
+    case QV_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+}  // extern "C"

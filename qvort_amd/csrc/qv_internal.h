@@ -1,0 +1,127 @@
+// qv_internal.h -- device data layout and kernel launchers shared by the
+// translation units of libqvort_gpu.so (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/qvort_gpu.h"
+
+namespace qv {
+
+constexpr double kPi = 3.14159265358979324;  // cdata.f90:176
+constexpr int kWave = 64;
+constexpr int kMaxLevels = 32;      // two 64-bit key words, 16 levels each
+constexpr int kStackDepth = 40;     // per-wave DFS stack (levels + 1 <= 33)
+
+// structure-of-arrays vectors of the filament state, one array per axis
+struct Vec3 {
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+};
+
+// shells of the periodic images in the loop order of calc_velocity
+// (timestep.f90:283-286): (0,0,0) first, then p outer .. r inner
+struct Shells {
+  int n = 1;
+  double s[27][3] = {};
+};
+
+// tree nodes in DFS preorder, empty cells omitted (they contribute nothing
+// and are skipped by tree_walk, tree.f90:349).
+//   hot    cx, cy, cz, T      centroid and the acceptance threshold on d^2
+//   meta   skip, j            index after the subtree; leaf: 0-based slot j
+//                             excluded-test index (f(parray(1)%infront)%behind),
+//                             internal: -1
+//   mp     internal: circ(3), C(9) row-major C[i][k]; leaf: ghosti(3) of the point
+//   mp32   the same moments in FP32 (FP32C path)
+//   geo    width, geo_dist, and the exact-test flag (see tree build)
+struct Tree {
+  int64_t nnodes = 0;
+  int64_t cap = 0;
+  double4 *hot = nullptr;
+  int2 *meta = nullptr;
+  double *mp = nullptr;     // 12 per node
+  float *mp32 = nullptr;    // 12 per node
+  double2 *geo = nullptr;   // width, geo_dist
+  // build-only arrays (per node)
+  int32_t *first = nullptr;  // first sorted position
+  int32_t *count = nullptr;  // member count (root: pcount slots)
+  int8_t *level = nullptr;
+  int8_t *digit = nullptr;   // child index within the parent (0..7)
+  double *corner = nullptr;  // 3 per node
+};
+
+struct Ctx {
+  qv_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t pcount = 0, cap = 0;
+  int32_t nact = 0;
+  bool have_velocity = false;
+  // state (SoA), slot order
+  Vec3 x, gi, gb, gii, gbb, u, uli, ubs, u1, u2, u3;
+  int32_t *infront = nullptr, *behind = nullptr;
+  // staging for host <-> device copies of one (pcount,3) array
+  double *stage = nullptr;
+  // tree build
+  uint64_t *key0 = nullptr, *key1 = nullptr, *key0_s = nullptr;
+  int32_t *slot = nullptr, *slot_s = nullptr;  // sorted active slots
+  int32_t *nnode_of = nullptr, *node_off = nullptr;  // per sorted position
+  Vec3 xs, gs;  // positions and ghosts in sorted order
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int32_t *counters = nullptr;   // [0] nact, [1] error flags, ...
+  int64_t *evals = nullptr;
+  Tree tree;
+  // pinned host scalars
+  int32_t *h_counters = nullptr;
+  int64_t *h_evals = nullptr;
+  qv_timings timings{};
+  hipEvent_t ev[6] = {};
+};
+
+// error flags accumulated in counters[1]
+enum : int32_t { kFlagSingular = 1, kFlagDepth = 2 };
+
+Shells make_shells(const qv_params &p);
+
+// per-target accumulators of FP32 terms.  AccF64 folds every term into an
+// FP64 sum (on CDNA4 an FP64 add costs the same issue slot as an FP32 add,
+// so this is the cheapest compensated sum); AccKahan keeps the FP32 sum with
+// a Kahan-Babuska (Neumaier) compensation term.
+struct AccF64 {
+  double s0 = 0., s1 = 0., s2 = 0.;
+  __device__ inline void add(float a, float b, float c) {
+    s0 += (double)a; s1 += (double)b; s2 += (double)c;
+  }
+  __device__ inline void get(double &a, double &b, double &c) const { a = s0; b = s1; c = s2; }
+};
+
+struct AccKahan {
+  float s[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
+  __device__ inline void add1(int k, float v) {
+    const float t = s[k] + v;
+    // two-sum error term without a branch (Knuth): exact when no overflow
+    const float bp = t - s[k];
+    const float ap = t - bp;
+    e[k] += (s[k] - ap) + (v - bp);
+    s[k] = t;
+  }
+  __device__ inline void add(float a, float b, float c) { add1(0, a); add1(1, b); add1(2, c); }
+  __device__ inline void get(double &a, double &b, double &c) const {
+    a = (double)s[0] + (double)e[0];
+    b = (double)s[1] + (double)e[1];
+    c = (double)s[2] + (double)e[2];
+  }
+};
+
+// launchers (each returns hipError_t of the launch)
+hipError_t launch_aos_to_soa(Ctx &c, const double *d_aos, Vec3 dst);
+hipError_t launch_soa_to_aos(Ctx &c, Vec3 src, double *d_aos);
+hipError_t launch_ghost_lia(Ctx &c);
+hipError_t launch_ab_update(Ctx &c);
+hipError_t launch_enforce_periodic(Ctx &c);
+hipError_t launch_direct(Ctx &c);
+int tree_build(Ctx &c);          // returns qv_status
+hipError_t launch_walk(Ctx &c);
+
+}  // namespace qv

@@ -1,0 +1,288 @@
+// qv_pointwise.hip -- per-slot kernels of the velocity stage: ghost gather,
+// local (LIA) term, Adams-Bashforth ladder, periodic wrap, layout transposes.
+//
+// All FP64 arithmetic keeps the reference's operation order and is compiled
+// without contraction, so these kernels reproduce the flang build of the
+// reference bit for bit (log() aside, which may differ in the last ulp).
+#include "qv_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace qv {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline double sq(double a) { return a * a; }
+
+// dist_gen, derivatives.f90:79-86
+__device__ inline double dist_gen(double ax, double ay, double az, double bx,
+                                  double by, double bz) {
+  return sqrt((sq(ax - bx) + sq(ay - by)) + sq(az - bz));
+}
+
+__global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict__ x,
+                             double *__restrict__ y, double *__restrict__ z, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = aos[3 * i + 0];
+  y[i] = aos[3 * i + 1];
+  z[i] = aos[3 * i + 2];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, double *__restrict__ aos, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  aos[3 * i + 0] = x[i];
+  aos[3 * i + 1] = y[i];
+  aos[3 * i + 2] = z[i];
+}
+
+// periodic unwrap of one ghost coordinate (periodic.f90:36-55 and repeats)
+__device__ inline double unwrap(double xi, double g, double L, bool on) {
+  if (!on) return g;
+  if ((xi - g) > (L / 2.)) return g + L;
+  if ((xi - g) < (-L / 2.)) return g - L;
+  return g;
+}
+
+struct Lia {
+  double qc, corea, L;
+  int order;
+  bool px, py, pz;  // periodic axes
+};
+
+// get_deriv_1 / get_deriv_2, derivatives.f90:11-75
+__device__ void derivs(const Lia &P, const double x[3], const double gi[3],
+                       const double gb[3], const double gii[3], const double gbb[3],
+                       double s1[3], double s2[3]) {
+  const double di = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
+  const double db = dist_gen(x[0], x[1], x[2], gb[0], gb[1], gb[2]);
+  if (P.order != 4) {
+    const double den = (2. * db) * di;
+    for (int k = 0; k < 3; ++k) s1[k] = ((db * gi[k] + (di - db) * x[k]) - di * gb[k]) / den;
+    for (int k = 0; k < 3; ++k)
+      s2[k] = 2. * ((gi[k] / (di * (di + db)) - x[k] / (di * db)) + gb[k] / (db * (di + db)));
+    return;
+  }
+  const double dii = dist_gen(gii[0], gii[1], gii[2], gi[0], gi[1], gi[2]);
+  const double dbb = dist_gen(gbb[0], gbb[1], gbb[2], gb[0], gb[1], gb[2]);
+  {
+    double A = db * (di * di) + (db * di) * dii;
+    A = A / (((dbb * (dbb + db)) * ((dbb + db) + di)) * (((dbb + db) + di) + dii));
+    double B = ((-(dbb * (di * di)) - db * (di * di)) - (dbb * di) * dii) - (db * di) * dii;
+    B = B / (((dbb * db) * (db + di)) * ((db + di) + dii));
+    double D = (((dbb * db) * di + (db * db) * di) + (dbb * db) * dii) + (db * db) * dii;
+    D = D / (((di * dii) * (db + di)) * ((dbb + db) + di));
+    double E = -(di * (db * db)) - (dbb * db) * di;
+    E = E / (((dii * (di + dii)) * ((db + di) + dii)) * (((dbb + db) + di) + dii));
+    const double C = -(((A + B) + D) + E);
+    for (int k = 0; k < 3; ++k)
+      s1[k] = (((A * gbb[k] + B * gb[k]) + C * x[k]) + D * gi[k]) + E * gii[k];
+  }
+  {
+    double A = 2. * (((-((2. * db) * di) + di * di) - db * dii) + di * dii);
+    A = A / (((dbb * (dbb + db)) * ((dbb + db) + di)) * (((dbb + db) + di) + dii));
+    double B = 2. * ((((((2. * dbb) * di + (2. * db) * di) - di * di) + dbb * dii) + db * dii) -
+                     di * dii);
+    B = B / (((dbb * db) * (db + di)) * ((db + di) + dii));
+    double D = 2. * (((((-(dbb * db) - db * db) + dbb * di) + (2. * db) * di) + dbb * dii) +
+                     (2. * db) * dii);
+    D = D / (((di * dii) * (db + di)) * ((dbb + db) + di));
+    double E = 2. * (((dbb * db + db * db) - dbb * di) - (2. * db) * di);
+    E = E / (((dii * (di + dii)) * ((db + di) + dii)) * (((dbb + db) + di) + dii));
+    const double C = -(((A + B) + D) + E);
+    for (int k = 0; k < 3; ++k)
+      s2[k] = (((A * gbb[k] + B * gb[k]) + C * x[k]) + D * gi[k]) + E * gii[k];
+  }
+}
+
+// ghostp (periodic.f90:8-184) fused with the local term of calc_velocity
+// (timestep.f90:221-225 / 270-274): u_LI = Gamma/(4 pi) ln((1.21/a)
+// sqrt(l_i l_b)) s' x s''
+__global__ void __launch_bounds__(kBlock)
+k_ghost_lia(Lia P, int n, const double *__restrict__ X, const double *__restrict__ Y,
+            const double *__restrict__ Z, const int32_t *__restrict__ inf,
+            const int32_t *__restrict__ beh, double *__restrict__ gix,
+            double *__restrict__ giy, double *__restrict__ giz, double *__restrict__ gbx,
+            double *__restrict__ gby, double *__restrict__ gbz, double *__restrict__ giix,
+            double *__restrict__ giiy, double *__restrict__ giiz, double *__restrict__ gbbx,
+            double *__restrict__ gbby, double *__restrict__ gbbz, double *__restrict__ ulx,
+            double *__restrict__ uly, double *__restrict__ ulz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int fi = inf[i];
+  if (fi == 0) {
+    gix[i] = giy[i] = giz[i] = 0.;
+    gbx[i] = gby[i] = gbz[i] = 0.;
+    giix[i] = giiy[i] = giiz[i] = 0.;
+    gbbx[i] = gbby[i] = gbbz[i] = 0.;
+    ulx[i] = uly[i] = ulz[i] = 0.;
+    return;
+  }
+  const int a = fi - 1, b = beh[i] - 1;
+  const int aa = inf[a] - 1, bb = beh[b] - 1;
+  const double x[3] = {X[i], Y[i], Z[i]};
+  double gi[3] = {X[a], Y[a], Z[a]};
+  double gb[3] = {X[b], Y[b], Z[b]};
+  double gii[3] = {X[aa], Y[aa], Z[aa]};
+  double gbb[3] = {X[bb], Y[bb], Z[bb]};
+  const bool per[3] = {P.px, P.py, P.pz};
+  for (int k = 0; k < 3; ++k) {
+    gi[k] = unwrap(x[k], gi[k], P.L, per[k]);
+    gb[k] = unwrap(x[k], gb[k], P.L, per[k]);
+    gii[k] = unwrap(x[k], gii[k], P.L, per[k]);
+    gbb[k] = unwrap(x[k], gbb[k], P.L, per[k]);
+  }
+  gix[i] = gi[0]; giy[i] = gi[1]; giz[i] = gi[2];
+  gbx[i] = gb[0]; gby[i] = gb[1]; gbz[i] = gb[2];
+  giix[i] = gii[0]; giiy[i] = gii[1]; giiz[i] = gii[2];
+  gbbx[i] = gbb[0]; gbby[i] = gbb[1]; gbbz[i] = gbb[2];
+  double s1[3], s2[3];
+  derivs(P, x, gi, gb, gii, gbb, s1, s2);
+  const double li = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
+  const double lb = dist_gen(x[0], x[1], x[2], gb[0], gb[1], gb[2]);
+  const double beta = (P.qc / (4. * kPi)) * log((1.21 / P.corea) * sqrt(li * lb));
+  ulx[i] = beta * (s1[1] * s2[2] - s1[2] * s2[1]);
+  uly[i] = beta * (s1[2] * s2[0] - s1[0] * s2[2]);
+  ulz[i] = beta * (s1[0] * s2[1] - s1[1] * s2[0]);
+}
+
+__device__ inline double maxabs3(double a, double b, double c) {
+  double m = fabs(a);
+  if (fabs(b) > m) m = fabs(b);
+  if (fabs(c) > m) m = fabs(c);
+  return m;
+}
+
+// Adams-Bashforth ladder keyed on exact zeros of the history
+// (timestep.f90:48-71): Euler, AB2, AB3, AB4, then u3<-u2<-u1<-u
+__global__ void __launch_bounds__(kBlock)
+k_ab_update(int n, double dt, const int32_t *__restrict__ inf, double *__restrict__ X,
+            double *__restrict__ Y, double *__restrict__ Z, const double *__restrict__ ux,
+            const double *__restrict__ uy, const double *__restrict__ uz,
+            double *__restrict__ u1x, double *__restrict__ u1y, double *__restrict__ u1z,
+            double *__restrict__ u2x, double *__restrict__ u2y, double *__restrict__ u2z,
+            double *__restrict__ u3x, double *__restrict__ u3y, double *__restrict__ u3z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || inf[i] == 0) return;
+  double x[3] = {X[i], Y[i], Z[i]};
+  const double u[3] = {ux[i], uy[i], uz[i]};
+  const double a1[3] = {u1x[i], u1y[i], u1z[i]};
+  const double a2[3] = {u2x[i], u2y[i], u2z[i]};
+  const double a3[3] = {u3x[i], u3y[i], u3z[i]};
+  const double three_twos = 3. / 2., one_half = 1. / 2.;
+  const double twenty_three_twelve = 23. / 12., four_thirds = 4. / 3., five_twelths = 5. / 12.;
+  if (maxabs3(a1[0], a1[1], a1[2]) == 0.) {
+    for (int k = 0; k < 3; ++k) x[k] = x[k] + dt * u[k];
+  } else if (maxabs3(a2[0], a2[1], a2[2]) == 0.) {
+    for (int k = 0; k < 3; ++k) x[k] = (x[k] + (three_twos * dt) * u[k]) - (one_half * dt) * a1[k];
+  } else if (maxabs3(a3[0], a3[1], a3[2]) == 0.) {
+    for (int k = 0; k < 3; ++k)
+      x[k] = ((x[k] + (twenty_three_twelve * dt) * u[k]) - (four_thirds * dt) * a1[k]) +
+             (five_twelths * dt) * a2[k];
+  } else {
+    for (int k = 0; k < 3; ++k)
+      x[k] = x[k] + (dt * (((55. * u[k] - 59. * a1[k]) + 37. * a2[k]) - 9. * a3[k])) / 24.;
+  }
+  X[i] = x[0]; Y[i] = x[1]; Z[i] = x[2];
+  u3x[i] = a2[0]; u3y[i] = a2[1]; u3z[i] = a2[2];
+  u2x[i] = a1[0]; u2y[i] = a1[1]; u2z[i] = a1[2];
+  u1x[i] = u[0]; u1y[i] = u[1]; u1z[i] = u[2];
+}
+
+// enforce_periodic / _yz / _z wrap part (periodic.f90:188-276); loop removal
+// of the open axes is host topology and not done here
+__global__ void k_wrap(int n, double L, int k0, const int32_t *__restrict__ inf,
+                       double *__restrict__ X, double *__restrict__ Y, double *__restrict__ Z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || inf[i] == 0) return;
+  double *A[3] = {X, Y, Z};
+  for (int k = k0; k < 3; ++k) {
+    const double v = A[k][i];
+    if (v > (L / 2.)) A[k][i] = v - L;
+    else if (v < (-L / 2.)) A[k][i] = v + L;
+  }
+}
+
+inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+Shells make_shells(const qv_params &p) {
+  Shells s;
+  const double L = p.box_size;
+  s.n = 1;
+  s.s[0][0] = s.s[0][1] = s.s[0][2] = 0.;
+  auto add = [&](int a, int b, int c) {
+    s.s[s.n][0] = a * L;
+    s.s[s.n][1] = b * L;
+    s.s[s.n][2] = c * L;
+    ++s.n;
+  };
+  if (p.boundary == QV_BC_PERIODIC) {
+    for (int a = -1; a <= 1; ++a)
+      for (int b = -1; b <= 1; ++b)
+        for (int c = -1; c <= 1; ++c)
+          if (a || b || c) add(a, b, c);
+  } else if (p.boundary == QV_BC_OPENX) {
+    for (int b = -1; b <= 1; ++b)
+      for (int c = -1; c <= 1; ++c)
+        if (b || c) add(0, b, c);
+  } else if (p.boundary == QV_BC_OPENXY) {
+    for (int c = -1; c <= 1; ++c)
+      if (c) add(0, 0, c);
+  }
+  return s;
+}
+
+hipError_t launch_aos_to_soa(Ctx &c, const double *d_aos, Vec3 dst) {
+  if (c.pcount == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, d_aos, dst.x,
+                     dst.y, dst.z, c.pcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(Ctx &c, Vec3 src, double *d_aos) {
+  if (c.pcount == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, src.x, src.y,
+                     src.z, d_aos, c.pcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_lia(Ctx &c) {
+  Lia P;
+  P.qc = c.p.quant_circ;
+  P.corea = c.p.corea;
+  P.L = c.p.box_size;
+  P.order = c.p.deriv_order;
+  P.px = c.p.boundary == QV_BC_PERIODIC;
+  P.py = c.p.boundary == QV_BC_PERIODIC || c.p.boundary == QV_BC_OPENX;
+  P.pz = c.p.boundary != QV_BC_OPEN;
+  hipLaunchKernelGGL(k_ghost_lia, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, P, c.pcount,
+                     c.x.x, c.x.y, c.x.z, c.infront, c.behind, c.gi.x, c.gi.y, c.gi.z, c.gb.x,
+                     c.gb.y, c.gb.z, c.gii.x, c.gii.y, c.gii.z, c.gbb.x, c.gbb.y, c.gbb.z,
+                     c.uli.x, c.uli.y, c.uli.z);
+  return hipGetLastError();
+}
+
+hipError_t launch_ab_update(Ctx &c) {
+  hipLaunchKernelGGL(k_ab_update, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
+                     c.p.dt, c.infront, c.x.x, c.x.y, c.x.z, c.u.x, c.u.y, c.u.z, c.u1.x, c.u1.y,
+                     c.u1.z, c.u2.x, c.u2.y, c.u2.z, c.u3.x, c.u3.y, c.u3.z);
+  return hipGetLastError();
+}
+
+hipError_t launch_enforce_periodic(Ctx &c) {
+  int k0 = 3;
+  if (c.p.boundary == QV_BC_PERIODIC) k0 = 0;
+  else if (c.p.boundary == QV_BC_OPENX) k0 = 1;
+  else if (c.p.boundary == QV_BC_OPENXY) k0 = 2;
+  if (k0 == 3) return hipSuccess;
+  hipLaunchKernelGGL(k_wrap, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
+                     c.p.box_size, k0, c.infront, c.x.x, c.x.y, c.x.z);
+  return hipGetLastError();
+}
+
+}  // namespace qv

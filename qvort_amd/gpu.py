@@ -1,0 +1,208 @@
+"""ctypes binding of libqvort_gpu.so (C ABI: include/qvort_gpu.h).
+
+The names follow the reference's hot-path procedures: ``upload`` hands over
+f(:)%x / infront / behind (cdata.f90:20-41), ``velocity`` runs ghostp +
+construct_tree + the velocity loop of pmotion (timestep.f90:41-47),
+``ab_step`` the Adams-Bashforth ladder (timestep.f90:48-71), ``tree_dump``
+returns the tree node by node in tree_walk's order.  Errors raise
+:class:`QvortError` carrying the library's status; the singular-leaf case
+carries the reference's fatal_error message (tree.f90:373-375).
+
+There is no fallback: if the library is missing or no GPU is visible, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libqvort_gpu.so")
+
+VEL = {"BS": 1, "Tree": 2}
+BC = {"open": 0, "periodic": 1, "openx": 2, "openxy": 3}
+PREC = {"fp64": 0, "fp32c": 1}
+
+QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
+    QV_ERR_NO_DEVICE = range(7)
+
+EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
+           "qv_velocity", "qv_ab_step", "qv_enforce_periodic", "qv_step", "qv_download",
+           "qv_tree_dump", "qv_get_timings", "qv_status_string"]
+
+
+class QvParams(C.Structure):
+    _fields_ = [("quant_circ", C.c_double), ("corea", C.c_double), ("box_size", C.c_double),
+                ("delta", C.c_double), ("tree_theta", C.c_double), ("dt", C.c_double),
+                ("velocity", C.c_int32), ("boundary", C.c_int32),
+                ("tree_extra_correction", C.c_int32), ("deriv_order", C.c_int32),
+                ("precision", C.c_int32), ("reserved", C.c_int32)]
+
+
+class QvTimings(C.Structure):
+    _fields_ = [("ghost_lia_ms", C.c_float), ("build_ms", C.c_float), ("walk_ms", C.c_float),
+                ("total_ms", C.c_float), ("nodes", C.c_int64), ("evals", C.c_int64)]
+
+
+class QvortError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_string(status)} (status {status})")
+
+
+def params(*, quant_circ=9.97e-4, corea=8.244023e-9, box_size=0.1, delta=5e-4, tree_theta=0.0,
+           dt=1e-6, velocity="Tree", boundary="periodic", tree_extra_correction=True,
+           deriv_order=2, precision="fp64") -> QvParams:
+    """run.in parameters of the hot path; defaults are the reference's
+    (cdata.f90:196-416) where it has one."""
+    if isinstance(deriv_order, str):
+        deriv_order = {"second": 2, "fourth": 4}[deriv_order]
+    return QvParams(quant_circ, corea, box_size, delta, tree_theta, dt, VEL[velocity],
+                    BC[boundary], 1 if tree_extra_correction else 0, int(deriv_order),
+                    PREC[precision], 0)
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} missing: run `python -m qvort_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        d, i32, i64 = P(C.c_double), P(C.c_int32), P(C.c_int64)
+        vp = C.c_void_p
+        L.qv_create.argtypes = [P(QvParams), P(vp)]
+        L.qv_destroy.argtypes = [vp]
+        L.qv_set_params.argtypes = [vp, P(QvParams)]
+        L.qv_upload.argtypes = [vp, C.c_int32, d, i32, i32]
+        L.qv_upload_history.argtypes = [vp, d, d, d]
+        L.qv_velocity.argtypes = [vp, d, d, d, i64]
+        L.qv_ab_step.argtypes = [vp]
+        L.qv_enforce_periodic.argtypes = [vp]
+        L.qv_step.argtypes = [vp, C.c_int32]
+        L.qv_download.argtypes = [vp, d, d, d, d, d]
+        L.qv_tree_dump.argtypes = [vp, d, C.c_int64, i64]
+        L.qv_get_timings.argtypes = [vp, P(QvTimings)]
+        L.qv_status_string.argtypes = [C.c_int]
+        L.qv_status_string.restype = C.c_char_p
+        for name in EXPORTS[:-1]:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def status_string(status: int) -> str:
+    try:
+        return lib().qv_status_string(status).decode()
+    except FileNotFoundError:
+        return "library missing"
+
+
+def _dp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != QV_OK:
+        raise QvortError(rc, where)
+
+
+class Context:
+    """One device context: f(:) state resident in HBM between calls."""
+
+    def __init__(self, p: QvParams):
+        self._p = p
+        self._h = C.c_void_p()
+        _check(lib().qv_create(C.byref(p), C.byref(self._h)), "qv_create")
+        self.pcount = 0
+
+    def close(self) -> None:
+        if self._h:
+            lib().qv_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: QvParams) -> None:
+        _check(lib().qv_set_params(self._h, C.byref(p)), "qv_set_params")
+        self._p = p
+
+    def upload(self, x, infront, behind) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        inf = np.ascontiguousarray(infront, dtype=np.int32)
+        beh = np.ascontiguousarray(behind, dtype=np.int32)
+        n = inf.shape[0]
+        if x.shape != (n, 3) or beh.shape != (n,):
+            raise ValueError("x must be (pcount,3), links (pcount,)")
+        _check(lib().qv_upload(self._h, n, _dp(x), _ip(inf), _ip(beh)), "qv_upload")
+        self.pcount = n
+
+    def upload_filaments(self, fil) -> None:
+        self.upload(fil.x, fil.infront, fil.behind)
+
+    def upload_history(self, u1=None, u2=None, u3=None) -> None:
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (u1, u2, u3)]
+        _check(lib().qv_upload_history(self._h, *[_dp(a) for a in arrs]), "qv_upload_history")
+
+    def velocity(self):
+        """(u, u_s_LI, u_s_BS, evals) of every slot."""
+        n = self.pcount
+        u, uli, ubs = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        ev = C.c_int64(0)
+        _check(lib().qv_velocity(self._h, _dp(u), _dp(uli), _dp(ubs), C.byref(ev)),
+               "qv_velocity")
+        return u, uli, ubs, int(ev.value)
+
+    def velocity_only(self) -> None:
+        """Velocity stage on the device without copying results back."""
+        _check(lib().qv_velocity(self._h, None, None, None, None), "qv_velocity")
+
+    def ab_step(self) -> None:
+        _check(lib().qv_ab_step(self._h), "qv_ab_step")
+
+    def enforce_periodic(self) -> None:
+        _check(lib().qv_enforce_periodic(self._h), "qv_enforce_periodic")
+
+    def step(self, nsteps: int) -> None:
+        _check(lib().qv_step(self._h, nsteps), "qv_step")
+
+    def download(self):
+        """(x, u, u1, u2, u3)."""
+        n = self.pcount
+        arrs = [np.zeros((n, 3)) for _ in range(5)]
+        _check(lib().qv_download(self._h, *[_dp(a) for a in arrs]), "qv_download")
+        return tuple(arrs)
+
+    def tree_dump(self) -> np.ndarray:
+        nn = C.c_int64(0)
+        _check(lib().qv_tree_dump(self._h, None, 0, C.byref(nn)), "qv_tree_dump")
+        out = np.zeros((nn.value, 22))
+        _check(lib().qv_tree_dump(self._h, _dp(out), nn.value, C.byref(nn)), "qv_tree_dump")
+        return out
+
+    def timings(self) -> dict:
+        t = QvTimings()
+        _check(lib().qv_get_timings(self._h, C.byref(t)), "qv_get_timings")
+        return {k: getattr(t, k) for k, _ in QvTimings._fields_}

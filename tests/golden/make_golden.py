@@ -1,0 +1,88 @@
+"""Write the golden vectors of tests/golden/*.npz from the reference build.
+
+Runs the flang build of the reference hot path (oracle/_ref/qvort_ref_driver,
+built by ``make -C oracle ref`` from /root/reference/src) on small seeded
+inputs and stores inputs and outputs as compressed numpy archives.  Each
+case records the run.in parameters it used.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import refrun  # noqa: E402
+from qvort_amd import geometry as G  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    loops = G.random_loops(0.1, 5e-4, n_loops=4, seed=11)
+    loops_open = G.random_loops(0.1, 5e-4, n_loops=4, seed=12, wrap=False)
+    holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=3, seed=13), 40, seed=5)
+    kel = G.kelvin_line(256, 0.1, amp_sigma=2e-3, seed=14)
+    ring = G.single_ring(512, 0.05)
+    base = dict(box_size=0.1, delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4)
+    rng = np.random.Generator(np.random.PCG64(99))
+    hist = [rng.normal(0, 0.05, size=loops.x.shape) for _ in range(3)]
+    return [
+        ("ring512_bs_open", ring, dict(base, velocity="BS", tree_theta=0.0, boundary="open",
+                                       box_size=0.2, delta=6.2e-4, dt=1e-5), 20, False, None),
+        ("loops_tree_periodic_t04", loops, dict(base, velocity="Tree", tree_theta=0.4,
+                                                boundary="periodic"), 3, True, None),
+        ("loops_bs_periodic", loops, dict(base, velocity="BS", tree_theta=0.0,
+                                          boundary="periodic"), 3, False, None),
+        ("loops_tree_open_t06", loops_open, dict(base, velocity="Tree", tree_theta=0.6,
+                                                 boundary="open"), 2, True, None),
+        ("loops_tree_openx_t03", loops, dict(base, velocity="Tree", tree_theta=0.3,
+                                             boundary="openx"), 2, False, None),
+        ("loops_bs_openxy", loops, dict(base, velocity="BS", tree_theta=0.0,
+                                        boundary="openxy"), 2, False, None),
+        ("loops_tree_fourth", loops, dict(base, velocity="Tree", tree_theta=0.4,
+                                          boundary="periodic", deriv_order="fourth"), 2, False,
+         None),
+        ("loops_tree_noextra", loops, dict(base, velocity="Tree", tree_theta=0.5,
+                                           boundary="periodic", tree_extra_correction=False), 2,
+         False, None),
+        ("holes_tree_periodic", holes, dict(base, velocity="Tree", tree_theta=0.4,
+                                            boundary="periodic"), 2, True, None),
+        ("kelvin_tree_periodic", kel, dict(base, velocity="Tree", tree_theta=0.4,
+                                           boundary="periodic"), 3, False, None),
+        ("loops_ab4_history", loops, dict(base, velocity="BS", tree_theta=0.0,
+                                          boundary="periodic"), 2, False, hist),
+    ]
+
+
+def main():
+    index = {}
+    for name, fil, params, nsteps, dump, hist in cases():
+        u1, u2, u3 = hist if hist is not None else (None, None, None)
+        r = refrun.run(fil, nsteps=nsteps, dump_tree=dump, threads=1, u1=u1, u2=u2, u3=u3,
+                       **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind, u=r["u"],
+                      u_li=r["u_li"], u_bs=r["u_bs"], ghosti=r["ghosti"], ghostb=r["ghostb"],
+                      x_final=r["x"], u1_final=r["u1"], u2_final=r["u2"], u3_final=r["u3"],
+                      evals=np.array([r["evals"]], dtype=np.int64))
+        if hist is not None:
+            arrays.update(u1_0=u1, u2_0=u2, u3_0=u3)
+        if dump:
+            arrays["tree"] = r["tree"]
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount, n_active=fil.n_active)
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "evals", r["evals"], flush=True)
+    with open(os.path.join(OUT, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
