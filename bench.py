@@ -1,0 +1,184 @@
+"""Benchmark of the filament velocity stage on one MI355X.
+
+Metric: point-velocities/s at 1M filament points, tree_theta = 0.4, FP32 far
+field with compensated sums against the plain FP64 path.  One point-velocity
+is the full velocity of one active point (FP64 LIA + the nonlocal sum over
+all 27 periodic image shells); a step is one velocity stage: ghost gather +
+LIA, octree build (keys, sort, nodes, moments) and the tree walk.
+
+Workload: seeded random loops (radius U[0.005, 0.02] cm, uniform centres and
+orientations, spacing 0.8 delta) filling a periodic cube L = 0.2 cm up to
+1e6 points, delta = 5e-4 cm, Gamma = 9.97e-4 cm^2/s, a = 1.3e-8 cm,
+tree_extra_correction on, second-order derivatives.  Inputs are uploaded to
+HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--points N]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "point-velocities/s @1M pts tree θ=0.4: FP32+comp vs FP64 GPU, rel err≤1e-5"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(n_points: int, seed: int = 20261202):
+    from qvort_amd import geometry as G
+    return G.random_loops(0.2, 5e-4, target_points=n_points, seed=seed, spacing=0.8)
+
+
+def run_precision(fil, precision: str, theta: float, steps: int, warmup: int):
+    from qvort_amd import gpu
+    p = gpu.params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
+                   delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6,
+                   precision=precision)
+    with gpu.Context(p) as c:
+        c.upload_filaments(fil)
+        for _ in range(warmup):
+            c.velocity_only()
+        walk_ms, build_ms, total_ms, gl_ms = [], [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c.velocity_only()
+            t = c.timings()
+            walk_ms.append(t["walk_ms"])
+            build_ms.append(t["build_ms"])
+            total_ms.append(t["total_ms"])
+            gl_ms.append(t["ghost_lia_ms"])
+        t1 = time.perf_counter()
+        tim = c.timings()
+        u, _, _, _ = c.velocity()
+        # the AB update is not part of the metric; timed on its own
+        ta = time.perf_counter()
+        c.ab_step()
+        ab_ms = (time.perf_counter() - ta) * 1e3
+    return dict(wall_s=t1 - t0, walk_ms=float(np.mean(walk_ms)), build_ms=float(np.mean(build_ms)),
+                device_ms=float(np.mean(total_ms)), ghost_lia_ms=float(np.mean(gl_ms)),
+                ab_ms=ab_ms, timings=tim, u=u)
+
+
+def cpu_baseline(fil, theta: float, n_sample: int):
+    """The C restatement (oracle, OpenMP) on this host: full octree build of the
+    same 1M-point input plus the walk of a strided sample of targets."""
+    from oracle import port
+    p = port.make_params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
+                         corea=1.3e-8)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    act = np.flatnonzero(fil.infront)
+    stride = max(1, act.size // n_sample)
+    targets = act[::stride][:n_sample]
+    _, _, build_s, walk_s = port.velocity_targets(p, fil, targets, nthreads=nthreads)
+    n = act.size
+    t_stage = build_s + walk_s * n / targets.size
+    return dict(value=n / t_stage, unit="point-velocities/s", cores=nthreads, kind="port",
+                sample=(f"oracle (C, OpenMP {nthreads} threads): full octree build of the same "
+                        f"{n}-point input ({build_s:.2f} s) + walk of {targets.size} strided "
+                        f"targets ({walk_s:.2f} s), stage time extrapolated to all targets"))
+
+
+def roofline(res, precision: str, n: int, traffic):
+    """Walk kernel against the HBM roofline with the algorithmic bytes of the
+    tree walk: 48 B per target (x in, u out, FP64) plus, per visited node 5
+    values (centroid, width, geo distance), per accepted cell 15 values
+    (centroid, circulation, C), per leaf 6 values (x_j, g_j), at b bytes per
+    value (8 for FP64, 4 for FP32)."""
+    t = res["timings"]
+    b = 8 if precision == "fp64" else 4
+    visits, leaves, cells = t["visits"], t["leaf_evals"], t["cell_evals"]
+    bytes_alg = n * 48 + b * (5 * visits + 15 * cells + 6 * leaves)
+    achieved = bytes_alg / (res["walk_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"k_walk<{precision}>", "bytes_alg_per_launch": int(bytes_alg),
+            "walk_ms": round(res["walk_ms"], 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--theta", type=float, default=0.4)
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the velocity "
+              "stage would shard over targets with a replicated tree, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    fil = workload(args.points)
+    n = fil.n_active
+    print(f"workload: {n} points, {len(fil.loop_len)} loops", file=sys.stderr, flush=True)
+    r64 = run_precision(fil, "fp64", args.theta, args.steps, args.warmup)
+    print(f"fp64: {r64['wall_s'] / args.steps * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+    r32 = run_precision(fil, "fp32c", args.theta, args.steps, args.warmup)
+    print(f"fp32c: {r32['wall_s'] / args.steps * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+
+    act = fil.infront != 0
+    nr = np.linalg.norm(r64["u"][act], axis=1)
+    e = np.linalg.norm(r32["u"][act] - r64["u"][act], axis=1) / nr
+    err = {"vs": "fp64 GPU, same tree", "max": float(e.max()),
+           "rms": float(np.sqrt(np.mean(e ** 2))),
+           "max_over_rms_velocity": float(np.max(np.linalg.norm(
+               r32["u"][act] - r64["u"][act], axis=1)) / np.sqrt(np.mean(nr ** 2)))}
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "walk_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("fp32c_bytes_per_launch")
+
+    ms32 = r32["wall_s"] / args.steps * 1e3
+    ms64 = r64["wall_s"] / args.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": n / (ms32 * 1e-3),
+        "unit": "point-velocities/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "random loops, periodic cube L=0.2 cm, delta=5e-4 cm",
+                   "n_points": n, "tree_theta": args.theta, "shells": 27,
+                   "precision": "fp32 far field, compensated sums; fp64 LIA and decisions",
+                   "parallelism": "1 GPU"},
+        "fp64": {"value": n / (ms64 * 1e-3), "ms_per_step": round(ms64, 3),
+                 "device_ms": round(r64["device_ms"], 3), "build_ms": round(r64["build_ms"], 3),
+                 "walk_ms": round(r64["walk_ms"], 3), "ghost_lia_ms": round(r64["ghost_lia_ms"], 3),
+                 "ab_ms": round(r64["ab_ms"], 3)},
+        "fp32c": {"device_ms": round(r32["device_ms"], 3), "build_ms": round(r32["build_ms"], 3),
+                  "walk_ms": round(r32["walk_ms"], 3),
+                  "ghost_lia_ms": round(r32["ghost_lia_ms"], 3), "ab_ms": round(r32["ab_ms"], 3)},
+        "fp32c_speedup_over_fp64": round(ms64 / ms32, 3),
+        "error": err,
+        "evals_per_point": r64["timings"]["evals"] / n,
+        "tree_nodes": r64["timings"]["nodes"],
+        "roofline": roofline(r32, "fp32c", n, traffic),
+        "roofline_fp64": roofline(r64, "fp64", n, None),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(fil, args.theta, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

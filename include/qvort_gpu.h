@@ -87,6 +87,9 @@ typedef struct qv_timings {
   float total_ms;      /* whole qv_velocity on the device */
   int64_t nodes;       /* non-empty tree nodes (Tree only) */
   int64_t evals;       /* interactions (reference eval_counter, 64-bit) */
+  int64_t visits;      /* node visits of all walks (Tree only) */
+  int64_t leaf_evals;  /* leaf segment interactions (BS: source pairs) */
+  int64_t cell_evals;  /* accepted-cell (multipole) interactions */
 } qv_timings;
 
 int qv_create(const qv_params *params, qv_ctx **out);

@@ -97,7 +97,7 @@ int velocity_stage(Ctx &c) {
   if (c.pcount == 0) return QV_ERR_STATE;
   TRY(hipEventRecord(c.ev[0], c.stream));
   TRY(hipMemsetAsync(c.counters, 0, 4 * sizeof(int32_t), c.stream));
-  TRY(hipMemsetAsync(c.evals, 0, sizeof(int64_t), c.stream));
+  TRY(hipMemsetAsync(c.evals, 0, 4 * sizeof(int64_t), c.stream));
   TRY(launch_ghost_lia(c));
   TRY(hipEventRecord(c.ev[1], c.stream));
   int rc = QV_OK;
@@ -115,7 +115,7 @@ int velocity_stage(Ctx &c) {
   TRY(hipEventRecord(c.ev[3], c.stream));
   TRY(hipMemcpyAsync(c.h_counters, c.counters, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
                      c.stream));
-  TRY(hipMemcpyAsync(c.h_evals, c.evals, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  TRY(hipMemcpyAsync(c.h_evals, c.evals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
   TRY(hipStreamSynchronize(c.stream));
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
@@ -131,9 +131,15 @@ int velocity_stage(Ctx &c) {
     // eval_counter starts at 2N (tree.f90:77-78) and counts every leaf and
     // accepted-cell interaction
     c.timings.evals = 2 * (int64_t)c.nact + c.h_evals[0];
+    c.timings.visits = c.h_evals[1];
+    c.timings.leaf_evals = c.h_evals[2];
+    c.timings.cell_evals = c.h_evals[0] - c.h_evals[2];
   } else {
     const int64_t S = make_shells(c.p).n;
     c.timings.evals = S * (int64_t)c.nact * (int64_t)c.nact;
+    c.timings.visits = 0;
+    c.timings.leaf_evals = c.timings.evals;
+    c.timings.cell_evals = 0;
   }
   if (c.h_counters[1] & kFlagSingular) return QV_ERR_SINGULAR;
   c.have_velocity = true;
@@ -145,7 +151,7 @@ struct DumpCtx {
   const std::vector<int32_t> &count;
   const std::vector<double> &corner, &mp;
   const std::vector<double2> &geo;
-  const std::vector<double4> &hot;
+  const std::vector<NodeRec> &rec;
   const std::vector<int2> &meta;
   double *out;
   int64_t max_nodes, k;
@@ -168,7 +174,7 @@ void dump_emit(DumpCtx &D, int lev, int pc, int leafj, const double *pos, double
 // recursion depth is at most 33
 void dump_rec(DumpCtx &D, int64_t n) {
   static const double zero[9] = {0};
-  const double cent[3] = {D.hot[n].x, D.hot[n].y, D.hot[n].z};
+  const double cent[3] = {D.rec[n].cx, D.rec[n].cy, D.rec[n].cz};
   const double *pos = &D.corner[3 * n];
   const double w = D.geo[n].x;
   if (D.meta[n].y >= 0) {
@@ -224,9 +230,9 @@ int qv_create(const qv_params *params, qv_ctx **out) {
     }
   }
   if (hipMalloc(&c->counters, 8 * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->evals, 2 * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&c->evals, 4 * sizeof(int64_t)) != hipSuccess ||
       hipHostMalloc(&c->h_counters, 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&c->h_evals, 2 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(&c->h_evals, 4 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
     qv_destroy(reinterpret_cast<qv_ctx *>(c));
     return QV_ERR_HIP;
   }
@@ -244,7 +250,8 @@ int qv_destroy(qv_ctx *ctx) {
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
                 c->slot_s, c->nnode_of, c->node_off, c->sort_tmp, c->counters, c->evals,
-                c->tree.hot, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
+                c->big_buf, c->partial,
+                c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -384,7 +391,7 @@ int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes) {
   std::vector<int32_t> count(nn);
   std::vector<double> corner(3 * nn), mp(12 * nn);
   std::vector<double2> geo(nn);
-  std::vector<double4> hot(nn);
+  std::vector<NodeRec> rec(nn);
   std::vector<int2> meta(nn);
   TRY(hipMemcpy(level.data(), c->tree.level, nn, hipMemcpyDeviceToHost));
   TRY(hipMemcpy(digit.data(), c->tree.digit, nn, hipMemcpyDeviceToHost));
@@ -392,11 +399,11 @@ int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes) {
   TRY(hipMemcpy(corner.data(), c->tree.corner, nn * 24, hipMemcpyDeviceToHost));
   TRY(hipMemcpy(mp.data(), c->tree.mp, nn * 96, hipMemcpyDeviceToHost));
   TRY(hipMemcpy(geo.data(), c->tree.geo, nn * 16, hipMemcpyDeviceToHost));
-  TRY(hipMemcpy(hot.data(), c->tree.hot, nn * 32, hipMemcpyDeviceToHost));
+  TRY(hipMemcpy(rec.data(), c->tree.rec, nn * sizeof(NodeRec), hipMemcpyDeviceToHost));
   TRY(hipMemcpy(meta.data(), c->tree.meta, nn * 8, hipMemcpyDeviceToHost));
   // expand to the reference's node list: every split cell has 8 children,
   // empty ones included, in DFS order (tree.f90:89-114)
-  DumpCtx D{level, digit, count, corner, mp, geo, hot, meta, out, max_nodes, 0};
+  DumpCtx D{level, digit, count, corner, mp, geo, rec, meta, out, max_nodes, 0};
   dump_rec(D, 0);
   const int64_t k = D.k;
   if (nnodes) *nnodes = k;

@@ -25,22 +25,39 @@ struct Shells {
   double s[27][3] = {};
 };
 
+// one node as the walk reads it: a single 64-byte scalar load per visit
+//   cx, cy, cz   centroid (leaf: the point's x, since x/1 == x)
+//   T            acceptance threshold on the squared target distance
+//                (< 0: evaluate the full test; leaves: unused)
+//   skip         index of the node after this subtree (DFS preorder)
+//   j            leaf: 0-based slot j = f(parray(1)%infront)%behind, whose
+//                exclusion the walk tests; internal node: -1
+//   leaf:        g[3] = ghosti of the point (FP64)
+//   internal:    m6[0..2] = circ, m6[3..5] = eps:C (FP32 copies)
+struct alignas(64) NodeRec {
+  double cx, cy, cz, T;
+  int32_t skip, j;
+  union {
+    double g[3];
+    float m6[6];
+  };
+};
+static_assert(sizeof(NodeRec) == 64, "node record is one 64-byte line");
+
 // tree nodes in DFS preorder, empty cells omitted (they contribute nothing
 // and are skipped by tree_walk, tree.f90:349).
-//   hot    cx, cy, cz, T      centroid and the acceptance threshold on d^2
-//   meta   skip, j            index after the subtree; leaf: 0-based slot j
-//                             excluded-test index (f(parray(1)%infront)%behind),
-//                             internal: -1
-//   mp     internal: circ(3), C(9) row-major C[i][k]; leaf: ghosti(3) of the point
-//   mp32   the same moments in FP32 (FP32C path)
-//   geo    width, geo_dist, and the exact-test flag (see tree build)
+//   rec    the walk's record (above)
+//   meta   skip, j (as in rec; used while building)
+//   mp     internal: circ(3), C(9) row-major C[i][k]; leaf: ghosti(3)
+//   mp32   internal: C(9) in FP32 (FP32C path)
+//   geo    width, geo_dist
 struct Tree {
   int64_t nnodes = 0;
   int64_t cap = 0;
-  double4 *hot = nullptr;
+  NodeRec *rec = nullptr;
   int2 *meta = nullptr;
   double *mp = nullptr;     // 12 per node
-  float *mp32 = nullptr;    // 12 per node
+  float *mp32 = nullptr;    // 16 per node
   double2 *geo = nullptr;   // width, geo_dist
   // build-only arrays (per node)
   int32_t *first = nullptr;  // first sorted position
@@ -69,7 +86,10 @@ struct Ctx {
   Vec3 xs, gs;  // positions and ghosts in sorted order
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  int32_t *counters = nullptr;   // [0] nact, [1] error flags, ...
+  int32_t *counters = nullptr;   // [1] error flags, [2] big nodes, [3] chunks
+  int32_t *big_buf = nullptr;    // big-node list, chunk bases, chunk owners
+  double *partial = nullptr;     // per-chunk partial sums of the big nodes
+  int64_t big_cap_ints = 0, partial_cap = 0;
   int64_t *evals = nullptr;
   Tree tree;
   // pinned host scalars

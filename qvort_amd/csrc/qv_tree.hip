@@ -276,12 +276,17 @@ struct MomArgs {
   const double *corner;
   const int2 *meta;
   double2 *geo;
-  double4 *hot;
+  NodeRec *rec;
   double *mp;
   float *mp32;
-  int32_t *big_list;
-  int32_t *big_count;
+  int32_t *big_list;   // big node ids (count > kSmallNode), in discovery order
+  int32_t *big_base;   // first chunk of each big node
+  int32_t *big_count;  // [0] big nodes, [1] chunks
+  int32_t *chunk_big;  // big-list index of each chunk
+  double *partial;     // 15 sums per chunk
 };
+
+constexpr int kChunk = 4096;  // members per workgroup in the big-node sums
 
 // tree_walk's acceptance test (tree.f90:351-365, 394) for a squared target
 // distance d2, evaluated in the reference's operation order
@@ -363,18 +368,20 @@ __device__ inline void finalize_node(const MomArgs &A, int64_t node, int n, doub
   if (meta.y >= 0) T = 0.;  // leaf: acceptance is never tested
   else if (is_root) T = -1.;
   else T = accept_threshold(w, geo, A.theta, A.extra);
-  A.hot[node] = make_double4(cx, cy, cz, T);
+  NodeRec r;
+  r.cx = cx; r.cy = cy; r.cz = cz; r.T = T;
+  r.skip = meta.x; r.j = meta.y;
+  r.m6[0] = (float)circ[0]; r.m6[1] = (float)circ[1]; r.m6[2] = (float)circ[2];
+  r.m6[3] = (float)(C[5] - C[7]);   // eps(:,k,l) C(k,l), the antisymmetric part
+  r.m6[4] = (float)(C[6] - C[2]);
+  r.m6[5] = (float)(C[1] - C[3]);
+  A.rec[node] = r;
   double *mp = A.mp + 12 * node;
   float *m32 = A.mp32 + 16 * node;
   for (int k = 0; k < 3; ++k) mp[k] = circ[k];
   for (int k = 0; k < 9; ++k) mp[3 + k] = C[k];
-  // FP32 copy: circ, eps:C (the antisymmetric part), C
-  m32[0] = (float)circ[0]; m32[1] = (float)circ[1]; m32[2] = (float)circ[2];
-  m32[3] = (float)(C[5] - C[7]);
-  m32[4] = (float)(C[6] - C[2]);
-  m32[5] = (float)(C[1] - C[3]);
-  for (int k = 0; k < 9; ++k) m32[6 + k] = (float)C[k];
-  m32[15] = 0.f;
+  for (int k = 0; k < 9; ++k) m32[k] = (float)C[k];
+  for (int k = 9; k < 16; ++k) m32[k] = 0.f;
 }
 
 __device__ inline void add_point(const MomArgs &A, int t, double s[15]) {
@@ -393,19 +400,19 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
   const bool is_root = node == 0;
   const int n = is_root ? A.nact : A.count[node];
   if (n > kSmallNode) {
+    // big node: its member range is reduced in chunks by k_moments_chunks
     const int q = atomicAdd(A.big_count, 1);
+    const int nch = (n + kChunk - 1) / kChunk;
+    const int base = atomicAdd(A.big_count + 1, nch);
     A.big_list[q] = (int)node;
+    A.big_base[q] = base;
+    for (int k = 0; k < nch; ++k) A.chunk_big[base + k] = q;
     return;
   }
   const int f = A.first[node];
   double s[15];
-  for (int k = 0; k < 15; ++k) s[k] = 0.;
-  if (n >= 1) {
-    // the leaf's sums are exact: s = x, e = g - x, g recovered as e + x
-    for (int t = f; t < f + n; ++t) add_point(A, t, s);
-  }
   if (n == 1 && !is_root) {
-    // keep x and g exactly (circ + cent must give back ghosti bit for bit)
+    // leaf: centroid x/1 == x, the record keeps x and ghosti exactly
     s[3] = A.gxs[f]; s[4] = A.gys[f]; s[5] = A.gzs[f];
     const double cent[3] = {A.xs[f], A.ys[f], A.zs[f]};
     const int2 meta = A.meta[node];
@@ -415,59 +422,87 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
                              (cent[1] - (P[1] + w / 2.)) * (cent[1] - (P[1] + w / 2.))) +
                             (cent[2] - (P[2] + w / 2.)) * (cent[2] - (P[2] + w / 2.)));
     A.geo[node].y = geo;
-    A.hot[node] = make_double4(cent[0], cent[1], cent[2], 0.);
-    // leaf record: ghosti (FP64) and the segment g - x (FP32)
+    NodeRec r;
+    r.cx = cent[0]; r.cy = cent[1]; r.cz = cent[2]; r.T = 0.;
+    r.skip = meta.x; r.j = meta.y;
+    r.g[0] = s[3]; r.g[1] = s[4]; r.g[2] = s[5];
+    A.rec[node] = r;
     double *mp = A.mp + 12 * node;
-    float *m32 = A.mp32 + 16 * node;
     mp[0] = s[3]; mp[1] = s[4]; mp[2] = s[5];
     for (int k = 3; k < 12; ++k) mp[k] = 0.;
-    for (int k = 0; k < 3; ++k) m32[k] = (float)(mp[k] - cent[k]);
-    for (int k = 3; k < 16; ++k) m32[k] = 0.f;
-    (void)meta;
     return;
   }
+  for (int k = 0; k < 15; ++k) s[k] = 0.;
+  for (int t = f; t < f + n; ++t) add_point(A, t, s);
   if (n == 0) {
-    A.hot[node] = make_double4(0., 0., 0., INFINITY);
+    NodeRec r;
+    r.cx = r.cy = r.cz = 0.;
+    r.T = INFINITY;
+    r.skip = A.meta[node].x;
+    r.j = A.meta[node].y;
+    r.g[0] = r.g[1] = r.g[2] = 0.;
+    A.rec[node] = r;
     return;
   }
   finalize_node(A, node, n, s, is_root);
 }
 
-// one workgroup per big node, deterministic tree reduction in LDS
-__global__ void __launch_bounds__(kBlock) k_moments_big(MomArgs A) {
-  __shared__ double red[15][kBlock];
-  const int nbig = *A.big_count;
-  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
-    const int64_t node = A.big_list[b];
-    const bool is_root = node == 0;
-    const int n = is_root ? A.nact : A.count[node];
+// one workgroup per chunk of a big node's member range; partial sums per
+// chunk, combined in chunk order by k_moments_big (deterministic)
+__global__ void __launch_bounds__(kBlock) k_moments_chunks(MomArgs A) {
+  __shared__ double red[15][kBlock / kWave];
+  const int nchunks = A.big_count[1];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int q = A.chunk_big[c];
+    const int64_t node = A.big_list[q];
+    const int n = node == 0 ? A.nact : A.count[node];
     const int f = A.first[node];
+    const int lo = f + (c - A.big_base[q]) * kChunk;
+    const int hi = min(f + n, lo + kChunk);
     double s[15];
     for (int k = 0; k < 15; ++k) s[k] = 0.;
-    for (int t = f + threadIdx.x; t < f + n; t += blockDim.x) add_point(A, t, s);
-    for (int k = 0; k < 15; ++k) red[k][threadIdx.x] = s[k];
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h)
-        for (int k = 0; k < 15; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < 15; ++k) s[k] = red[k][0];
-      finalize_node(A, node, n, s, is_root);
+    for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) add_point(A, t, s);
+    for (int k = 0; k < 15; ++k) {
+      double v = s[k];
+      for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[k][wid] = v;
     }
     __syncthreads();
+    if (threadIdx.x < 15) {
+      double v = 0.;
+      for (int w = 0; w < kBlock / kWave; ++w) v += red[threadIdx.x][w];
+      A.partial[15 * (int64_t)c + threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// one thread per big node: chunk partials in order, then the node record
+__global__ void __launch_bounds__(kBlock) k_moments_big(MomArgs A) {
+  const int nbig = A.big_count[0];
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nbig; q += gridDim.x * blockDim.x) {
+    const int64_t node = A.big_list[q];
+    const bool is_root = node == 0;
+    const int n = is_root ? A.nact : A.count[node];
+    const int nch = (n + kChunk - 1) / kChunk;
+    const int base = A.big_base[q];
+    double s[15];
+    for (int k = 0; k < 15; ++k) s[k] = 0.;
+    for (int c = base; c < base + nch; ++c)
+      for (int k = 0; k < 15; ++k) s[k] += A.partial[15 * (int64_t)c + k];
+    finalize_node(A, node, n, s, is_root);
   }
 }
 
 hipError_t grow_tree(Tree &t, int64_t need) {
   if (need <= t.cap) return hipSuccess;
   int64_t cap = need + need / 4 + 1024;
-  (void)hipFree(t.hot); (void)hipFree(t.meta); (void)hipFree(t.mp); (void)hipFree(t.mp32); (void)hipFree(t.geo);
+  (void)hipFree(t.rec); (void)hipFree(t.meta); (void)hipFree(t.mp); (void)hipFree(t.mp32); (void)hipFree(t.geo);
   (void)hipFree(t.first); (void)hipFree(t.count); (void)hipFree(t.level); (void)hipFree(t.digit); (void)hipFree(t.corner);
   t = Tree{};
   hipError_t e = hipSuccess;
-  if ((e = hipMalloc(&t.hot, cap * sizeof(double4)))) return e;
+  if ((e = hipMalloc(&t.rec, cap * sizeof(NodeRec)))) return e;
   if ((e = hipMalloc(&t.meta, cap * sizeof(int2)))) return e;
   if ((e = hipMalloc(&t.mp, cap * 12 * sizeof(double)))) return e;
   if ((e = hipMalloc(&t.mp32, cap * 16 * sizeof(float)))) return e;
@@ -584,14 +619,35 @@ int tree_build(Ctx &c) {
   MA.corner = c.tree.corner;
   MA.meta = c.tree.meta;
   MA.geo = c.tree.geo;
-  MA.hot = c.tree.hot;
+  MA.rec = c.tree.rec;
   MA.mp = c.tree.mp;
   MA.mp32 = c.tree.mp32;
-  MA.big_list = c.nnode_of;  // reused: per-position counts are no longer needed
+  // big nodes hold > kSmallNode members and are disjoint within a level:
+  // at most 33 * nact / 65 of them, and at most that many plus
+  // 33 * nact / kChunk chunks
+  const int64_t big_cap = 33 * (int64_t)nact / (kSmallNode + 1) + 1;
+  const int64_t chunk_cap = big_cap + 33 * (int64_t)nact / kChunk + 1;
+  if (big_cap * 2 + chunk_cap > c.big_cap_ints || chunk_cap > c.partial_cap) {
+    (void)hipFree(c.big_buf);
+    (void)hipFree(c.partial);
+    c.big_buf = nullptr;
+    c.partial = nullptr;
+    c.big_cap_ints = c.partial_cap = 0;
+    QV_TRY(hipMalloc(&c.big_buf, (big_cap * 2 + chunk_cap) * sizeof(int32_t)));
+    QV_TRY(hipMalloc(&c.partial, chunk_cap * 15 * sizeof(double)));
+    c.big_cap_ints = big_cap * 2 + chunk_cap;
+    c.partial_cap = chunk_cap;
+  }
+  MA.big_list = c.big_buf;
+  MA.big_base = c.big_buf + big_cap;
+  MA.chunk_big = c.big_buf + 2 * big_cap;
+  MA.partial = c.partial;
   MA.big_count = c.counters + 2;
   hipLaunchKernelGGL(k_moments_small, dim3(grid_of(nnodes)), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_moments_big, dim3(1024), dim3(kBlock), 0, st, MA);
+  hipLaunchKernelGGL(k_moments_chunks, dim3(2048), dim3(kBlock), 0, st, MA);
+  QV_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_moments_big, dim3(64), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
   return QV_OK;
 }
