@@ -87,21 +87,50 @@ def cpu_baseline(fil, theta: float, n_sample: int):
                         f"targets ({walk_s:.2f} s), stage time extrapolated to all targets"))
 
 
+FLOP_PEAK_TFS = {"fp32c": 157.3, "fp64": 78.6}  # MI355X published fp32 / fp64
+
+
 def roofline(res, precision: str, n: int, traffic):
-    """Walk kernel against the HBM roofline with the algorithmic bytes of the
-    tree walk: 48 B per target (x in, u out, FP64) plus, per visited node 5
-    values (centroid, width, geo distance), per accepted cell 15 values
-    (centroid, circulation, C), per leaf 6 values (x_j, g_j), at b bytes per
-    value (8 for FP64, 4 for FP32)."""
+    """Walk kernel against the compute roofline (vector ALU; the walk has no
+    dense contraction, so the MFMA peaks of the same precision are the
+    published ceiling).  Algorithmic flops, counting +,-,*,/,sqrt as one:
+    12 per node test (per target and visited node), 44 per leaf segment,
+    64 per accepted cell, with the counts taken from the kernel's own
+    64-bit counters.  Also reported: the HBM view of the same launch with
+    the per-target node bytes (48 B per target; per visit 5 values, per cell
+    15, per leaf 6, at 4 or 8 bytes), which exceeds the HBM peak because a
+    wavefront's scalar node loads are shared by its 64 targets."""
     t = res["timings"]
-    b = 8 if precision == "fp64" else 4
     visits, leaves, cells = t["visits"], t["leaf_evals"], t["cell_evals"]
+    flops = 12 * visits + 44 * leaves + 64 * cells
+    tflops = flops / (res["walk_ms"] * 1e-3) / 1e12
+    peak = FLOP_PEAK_TFS[precision]
+    b = 8 if precision == "fp64" else 4
     bytes_alg = n * 48 + b * (5 * visits + 15 * cells + 6 * leaves)
-    achieved = bytes_alg / (res["walk_ms"] * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"k_walk<{precision}>", "bytes_alg_per_launch": int(bytes_alg),
-            "walk_ms": round(res["walk_ms"], 3)}
+    return {"bound": "mfma", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(tflops / peak, 4), "traffic": traffic,
+            "kernel": f"k_walk<{precision}>", "flops_per_launch": int(flops),
+            "walk_ms": round(res["walk_ms"], 3),
+            "lane_utilisation": round(visits / (64.0 * max(1, t["wave_steps"])), 4),
+            "counts": {"visits": visits, "leaves": leaves, "cells": cells,
+                       "wave_steps": t["wave_steps"]},
+            "hbm_view": {"bytes_alg_per_launch": int(bytes_alg),
+                         "GB_per_s": round(bytes_alg / (res["walk_ms"] * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS}}
+
+
+def copy_calibration(gib: float = 2.0, reps: int = 5) -> None:
+    """A plain device copy of known size (torch), for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}))
 
 
 def main():
@@ -113,7 +142,14 @@ def main():
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32c"], default=None,
+                    help="profile one precision (no JSON line)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain copy of known size (for rocprofv3 counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the velocity "
               "stage would shard over targets with a replicated tree, see DESIGN.md",
@@ -123,6 +159,11 @@ def main():
     fil = workload(args.points)
     n = fil.n_active
     print(f"workload: {n} points, {len(fil.loop_len)} loops", file=sys.stderr, flush=True)
+    if args.only:
+        r = run_precision(fil, args.only, args.theta, args.steps, args.warmup)
+        print(f"{args.only}: {r['wall_s'] / args.steps * 1e3:.2f} ms/step, walk "
+              f"{r['walk_ms']:.3f} ms", file=sys.stderr, flush=True)
+        return
     r64 = run_precision(fil, "fp64", args.theta, args.steps, args.warmup)
     print(f"fp64: {r64['wall_s'] / args.steps * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
     r32 = run_precision(fil, "fp32c", args.theta, args.steps, args.warmup)
@@ -140,7 +181,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "walk_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("fp32c_bytes_per_launch")
+            traffic = json.load(fh).get("k_walk_fp32c_bytes_per_launch")
 
     ms32 = r32["wall_s"] / args.steps * 1e3
     ms64 = r64["wall_s"] / args.steps * 1e3

@@ -90,6 +90,7 @@ typedef struct qv_timings {
   int64_t visits;      /* node visits of all walks (Tree only) */
   int64_t leaf_evals;  /* leaf segment interactions (BS: source pairs) */
   int64_t cell_evals;  /* accepted-cell (multipole) interactions */
+  int64_t wave_steps;  /* node records fetched by the walk's wavefronts */
 } qv_timings;
 
 int qv_create(const qv_params *params, qv_ctx **out);

@@ -134,12 +134,14 @@ int velocity_stage(Ctx &c) {
     c.timings.visits = c.h_evals[1];
     c.timings.leaf_evals = c.h_evals[2];
     c.timings.cell_evals = c.h_evals[0] - c.h_evals[2];
+    c.timings.wave_steps = c.h_evals[3];
   } else {
     const int64_t S = make_shells(c.p).n;
     c.timings.evals = S * (int64_t)c.nact * (int64_t)c.nact;
     c.timings.visits = 0;
     c.timings.leaf_evals = c.timings.evals;
     c.timings.cell_evals = 0;
+    c.timings.wave_steps = 0;
   }
   if (c.h_counters[1] & kFlagSingular) return QV_ERR_SINGULAR;
   c.have_velocity = true;

@@ -49,7 +49,8 @@ struct WalkArgs {
   const double2 *geo;
   const double *ulx, *uly, *ulz;    // slot order
   double *ux, *uy, *uz, *bx, *by, *bz;
-  unsigned long long *evals;  // [0] leaf + cell interactions, [1] node visits, [2] leaves
+  unsigned long long *evals;  // [0] leaf + cell interactions, [1] lane visits, [2] leaves,
+                              // [3] wave steps (nodes fetched by a wavefront)
   int32_t *flags;
 };
 
@@ -74,24 +75,28 @@ __device__ inline bool accept_full(double d2, double w, double g, double theta, 
 }
 
 // leaf term, tree.f90:380-392, in the reference's operation order up to the
-// factor Gamma/(4 pi), which is applied once to the target's sum
-__device__ inline void leaf64(const double xi[3], double cx, double cy, double cz, double gx,
-                              double gy, double gz, const double s[3], double u[3]) {
+// factor Gamma/(4 pi), which is applied once to the target's sum.  Added to u
+// only where `on` (the other lanes may hold inf/nan terms).
+__device__ inline void leaf64(bool on, const double xi[3], double cx, double cy, double cz,
+                              double gx, double gy, double gz, const double s[3], double u[3]) {
   const double R1[3] = {(cx + s[0]) - xi[0], (cy + s[1]) - xi[1], (cz + s[2]) - xi[2]};
   const double R2[3] = {(gx + s[0]) - xi[0], (gy + s[1]) - xi[1], (gz + s[2]) - xi[2]};
   const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
   const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
   const double c = (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
   const double f = (a + b) / c;
-  u[0] = u[0] + f * (R1[1] * R2[2] - R1[2] * R2[1]);
-  u[1] = u[1] + f * (R1[2] * R2[0] - R1[0] * R2[2]);
-  u[2] = u[2] + f * (R1[0] * R2[1] - R1[1] * R2[0]);
+  const double w0 = f * (R1[1] * R2[2] - R1[2] * R2[1]);
+  const double w1 = f * (R1[2] * R2[0] - R1[0] * R2[2]);
+  const double w2 = f * (R1[0] * R2[1] - R1[1] * R2[0]);
+  u[0] = u[0] + (on ? w0 : 0.);
+  u[1] = u[1] + (on ? w1 : 0.);
+  u[2] = u[2] + (on ? w2 : 0.);
 }
 
 // cell term, tree.f90:400-411 (without Gamma/(4 pi)); r = x_i - (cent +
-// shift), d = |r|
-__device__ inline void cell64(const double r[3], double d2, const double *__restrict__ m,
-                              double u[3]) {
+// shift), d = |r|; added where `on`
+__device__ inline void cell64(bool on, const double r[3], double d2,
+                              const double *__restrict__ m, double u[3]) {
   const double d = sqrt(d2);
   const double d3 = (d * d) * d;
   const double d5 = (((d * d) * d) * d) * d;
@@ -104,53 +109,72 @@ __device__ inline void cell64(const double r[3], double d2, const double *__rest
   const double v0 = (r[0] * C[0] + r[1] * C[3]) + r[2] * C[6];
   const double v1 = (r[0] * C[1] + r[1] * C[4]) + r[2] * C[7];
   const double v2 = (r[0] * C[2] + r[1] * C[5]) + r[2] * C[8];
-  u[0] = u[0] + (((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5);
-  u[1] = u[1] + (((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5);
-  u[2] = u[2] + (((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5);
+  const double t0 = ((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5;
+  const double t1 = ((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5;
+  const double t2 = ((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5;
+  u[0] = u[0] + (on ? t0 : 0.);
+  u[1] = u[1] + (on ? t1 : 0.);
+  u[2] = u[2] + (on ? t2 : 0.);
 }
 
-// FP32 leaf: R1 = x_p + s - x_i and e = g_p - x_p formed in FP64, rounded once
-template <class Acc>
-__device__ inline void leaf32(double r0d, double r1d, double r2d, double e0d, double e1d,
-                              double e2d, Acc &acc) {
+// FP32 terms.  The parity code of this file is compiled without
+// contraction; the FP32 path has no reference rounding to keep, so it uses
+// explicit fused multiply-adds.
+__device__ inline float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// FP32 leaf: R1 = x_p + s - x_i and e = g_p - x_p formed in FP64, rounded once.
+// Straight-segment Biot-Savart (tree.f90:380-392) with R1 x R2 = R1 x e and,
+// for targets projecting inside the segment, ab + R1.R2 = |R1 x e|^2 /
+// (ab - R1.R2), which keeps FP32 free of cancellation.
+__device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, double e0d,
+                              double e1d, double e2d, AccF64 &acc) {
   const float r0 = (float)r0d, r1 = (float)r1d, r2 = (float)r2d;
   const float e0 = (float)e0d, e1 = (float)e1d, e2 = (float)e2d;
-  const float c0 = r1 * e2 - r2 * e1, c1 = r2 * e0 - r0 * e2, c2 = r0 * e1 - r1 * e0;
+  const float c0 = fmaf_(r1, e2, -r2 * e1), c1 = fmaf_(r2, e0, -r0 * e2),
+              c2 = fmaf_(r0, e1, -r1 * e0);
   const float s0 = r0 + e0, s1 = r1 + e1, s2 = r2 + e2;
-  const float a2 = (r0 * r0 + r1 * r1) + r2 * r2;
-  const float b2 = (s0 * s0 + s1 * s1) + s2 * s2;
+  const float a2 = fmaf_(r2, r2, fmaf_(r1, r1, r0 * r0));
+  const float b2 = fmaf_(s2, s2, fmaf_(s1, s1, s0 * s0));
   const float ia = __builtin_amdgcn_rsqf(a2), ib = __builtin_amdgcn_rsqf(b2);
   const float a = a2 * ia, b = b2 * ib;
   const float ab = a * b;
-  const float dt = (r0 * s0 + r1 * s1) + r2 * s2;
+  const float dt = fmaf_(r2, s2, fmaf_(r1, s1, r0 * s0));
   const float g = (a + b) * (ia * ib);
   float f;
   if (dt >= 0.f) {
     f = g * __builtin_amdgcn_rcpf(ab + dt);
   } else {
-    // target projects inside the segment: ab + R1.R2 = |R1 x e|^2 / (ab - R1.R2)
-    const float cc = (c0 * c0 + c1 * c1) + c2 * c2;
+    const float cc = fmaf_(c2, c2, fmaf_(c1, c1, c0 * c0));
     f = g * (ab - dt) * __builtin_amdgcn_rcpf(cc);
   }
-  acc.add(f * c0, f * c1, f * c2);
+  const double fd = on ? (double)f : 0.;
+  acc.s0 = fma(fd, (double)(on ? c0 : 0.f), acc.s0);
+  acc.s1 = fma(fd, (double)(on ? c1 : 0.f), acc.s1);
+  acc.s2 = fma(fd, (double)(on ? c2 : 0.f), acc.s2);
 }
 
-// FP32 cell: m6 = circ(3), eps:C(3) from the node record, C(9) from mp32
-template <class Acc>
-__device__ inline void cell32(const double rd[3], double d2, const float *m6,
-                              const float *__restrict__ C, Acc &acc) {
+// FP32 cell: m6 = circ(3), eps:C(3) from the node record, C(9) from mp32.
+// t = [(circ x r - eps:C) + 3 r x (C^T r) / d^2] / d^3, the 1/d^3 applied in
+// the FP64 accumulation.
+__device__ inline void cell32(bool on, const double rd[3], double d2, const float *m6,
+                              const float *__restrict__ C, AccF64 &acc) {
   const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
   const float id = __builtin_amdgcn_rsqf((float)d2);
   const float id2 = id * id;
-  const float id3 = id2 * id;
-  const float id5 = id3 * id2;
-  const float v0 = (r0 * C[0] + r1 * C[3]) + r2 * C[6];
-  const float v1 = (r0 * C[1] + r1 * C[4]) + r2 * C[7];
-  const float v2 = (r0 * C[2] + r1 * C[5]) + r2 * C[8];
-  const float t0 = ((m6[1] * r2 - m6[2] * r1) - m6[3]) * id3 + (3.f * id5) * (r1 * v2 - r2 * v1);
-  const float t1 = ((m6[2] * r0 - m6[0] * r2) - m6[4]) * id3 + (3.f * id5) * (r2 * v0 - r0 * v2);
-  const float t2 = ((m6[0] * r1 - m6[1] * r0) - m6[5]) * id3 + (3.f * id5) * (r0 * v1 - r1 * v0);
-  acc.add(t0, t1, t2);
+  const float q = 3.f * id2;
+  const double id3 = on ? (double)(id2 * id) : 0.;
+  const float v0 = fmaf_(r2, C[6], fmaf_(r1, C[3], r0 * C[0]));
+  const float v1 = fmaf_(r2, C[7], fmaf_(r1, C[4], r0 * C[1]));
+  const float v2 = fmaf_(r2, C[8], fmaf_(r1, C[5], r0 * C[2]));
+  const float X0 = fmaf_(m6[1], r2, fmaf_(-m6[2], r1, -m6[3]));
+  const float X1 = fmaf_(m6[2], r0, fmaf_(-m6[0], r2, -m6[4]));
+  const float X2 = fmaf_(m6[0], r1, fmaf_(-m6[1], r0, -m6[5]));
+  const float Y0 = fmaf_(r1, v2, -r2 * v1);
+  const float Y1 = fmaf_(r2, v0, -r0 * v2);
+  const float Y2 = fmaf_(r0, v1, -r1 * v0);
+  acc.s0 = fma((double)(on ? fmaf_(Y0, q, X0) : 0.f), id3, acc.s0);
+  acc.s1 = fma((double)(on ? fmaf_(Y1, q, X1) : 0.f), id3, acc.s1);
+  acc.s2 = fma((double)(on ? fmaf_(Y2, q, X2) : 0.f), id3, acc.s2);
 }
 
 struct alignas(16) StackEntry {
@@ -180,15 +204,26 @@ __global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
   }
   const uint64_t active = __ballot(valid);
   if (active == 0) return;
-  double uc[3] = {0., 0., 0.}, us[3] = {0., 0., 0.};
-  Acc ac, as;
-  uint32_t ev = 0, nvis = 0, nleaf = 0;
+  // one running sum: the central shell's is banked in uc after shell 0, the
+  // image shells then accumulate from zero (timestep.f90:276-288)
+  double uc[3] = {0., 0., 0.};
+  double u[3] = {0., 0., 0.};
+  Acc acc;
+  // per-wavefront counts (scalar): lane visits, leaf and cell interactions
+  uint32_t nvis = 0, nleaf = 0, ncell = 0, nstep = 0;
   bool singular = false;
   StackEntry *st = stack[wid];
   for (int sh = 0; sh < S.n; ++sh) {
     const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    double *u = sh == 0 ? uc : us;
-    Acc &acc = sh == 0 ? ac : as;
+    if (sh == 1) {
+      if (kMode == QV_FP64) {
+        uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2];
+        u[0] = u[1] = u[2] = 0.;
+      } else {
+        acc.get(uc[0], uc[1], uc[2]);
+        acc = Acc();
+      }
+    }
     // current entry (node, end of its sibling list, lanes) in SGPRs; the LDS
     // stack holds only sibling lists left behind when descending
     int n = 0, e = (int)A.nnodes;
@@ -196,45 +231,45 @@ __global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
     int sp = 0;
     while (true) {
       const NodeRec rec = A.rec[n];
+      ++nstep;
       const int skip = uni(rec.skip), j = uni(rec.j);
       const bool mine = (M >> lane) & 1ull;
-      nvis += mine ? 1u : 0u;
+      nvis += (uint32_t)__popcll(M);
       const double cs0 = rec.cx + sv[0], cs1 = rec.cy + sv[1], cs2 = rec.cz + sv[2];
       const double r[3] = {xi[0] - cs0, xi[1] - cs1, xi[2] - cs2};
       const double d2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
       uint64_t O = 0;
       if (j >= 0) {
         // leaf: the single point's segment unless it is i, behind(i), infront(i)
-        if (mine && j != t && j != bi && j != fi) {
-          if (d2 < 1e-30 && sqrt(d2) < 2.220446049250313e-16) singular = true;
+        const bool on = mine && j != t && j != bi && j != fi;
+        const uint64_t L = __ballot(on);
+        if (L) {
+          if (__ballot(on && d2 < 1e-30)) {
+            if (on && d2 < 1e-30 && sqrt(d2) < 2.220446049250313e-16) singular = true;
+          }
           if (kMode == QV_FP64) {
-            leaf64(xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
+            leaf64(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
           } else {
-            leaf32(-r[0], -r[1], -r[2], rec.g[0] - rec.cx, rec.g[1] - rec.cy,
+            leaf32(on, -r[0], -r[1], -r[2], rec.g[0] - rec.cx, rec.g[1] - rec.cy,
                    rec.g[2] - rec.cz, acc);
           }
-          ++ev;
-          ++nleaf;
+          nleaf += (uint32_t)__popcll(L);
         }
       } else {
-        bool open = false;
-        if (mine) {
-          bool acc_ok;
-          if (rec.T >= 0.) {
-            acc_ok = d2 >= rec.T;
-          } else {
-            const double2 g = A.geo[n];
-            acc_ok = accept_full(d2, g.x, g.y, A.theta, A.extra);
-          }
-          if (acc_ok) {
-            if (kMode == QV_FP64) cell64(r, d2, A.mp + 12 * (int64_t)n, u);
-            else cell32(r, d2, rec.m6, A.mp32 + 16 * (int64_t)n, acc);
-            ++ev;
-          } else {
-            open = true;
-          }
+        bool ok;
+        if (rec.T >= 0.) {
+          ok = mine && d2 >= rec.T;
+        } else {
+          const double2 g = A.geo[n];
+          ok = mine && accept_full(d2, g.x, g.y, A.theta, A.extra);
         }
-        O = __ballot(open);
+        const uint64_t Acc_ = __ballot(ok);
+        if (Acc_) {
+          if (kMode == QV_FP64) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
+          else cell32(ok, r, d2, rec.m6, A.mp32 + 16 * (int64_t)n, acc);
+          ncell += (uint32_t)__popcll(Acc_);
+        }
+        O = M & ~Acc_;
       }
       if (O != 0 && n + 1 != skip) {
         // descend into the children [n+1, skip) with the opening lanes
@@ -258,23 +293,25 @@ __global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
       }
     }
   }
-  // interaction, visit and leaf counts (64-bit) and the error flag
-  unsigned long long c0 = ev, c1 = nvis, c2 = nleaf;
-  for (int off = 32; off > 0; off >>= 1) {
-    c0 += __shfl_xor(c0, off);
-    c1 += __shfl_xor(c1, off);
-    c2 += __shfl_xor(c2, off);
-  }
+  // interaction, visit, leaf and step counts (64-bit) and the error flag
   if (lane == 0) {
-    atomicAdd(A.evals + 0, c0);
-    atomicAdd(A.evals + 1, c1);
-    atomicAdd(A.evals + 2, c2);
+    atomicAdd(A.evals + 0, (unsigned long long)nleaf + ncell);
+    atomicAdd(A.evals + 1, (unsigned long long)nvis);
+    atomicAdd(A.evals + 2, (unsigned long long)nleaf);
+    atomicAdd(A.evals + 3, (unsigned long long)nstep);
   }
   if (singular) atomicOr(A.flags, kFlagSingular);
   if (!valid) return;
-  if (kMode != QV_FP64) {
-    ac.get(uc[0], uc[1], uc[2]);
-    as.get(us[0], us[1], us[2]);
+  double us[3] = {0., 0., 0.};
+  if (kMode == QV_FP64) {
+    if (S.n == 1) {
+      uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2];
+    } else {
+      us[0] = u[0]; us[1] = u[1]; us[2] = u[2];
+    }
+  } else {
+    if (S.n == 1) acc.get(uc[0], uc[1], uc[2]);
+    else acc.get(us[0], us[1], us[2]);
   }
   // u_bs * quant_circ / 4 / pi (tree.f90:390, 411), once per sum
   for (int k = 0; k < 3; ++k) {

@@ -1,0 +1,137 @@
+!> iso_c_binding interface to libqvort_gpu.so (include/qvort_gpu.h).
+!>
+!> A Fortran driver keeps Qvort's f(:) arrays on the host in the layout
+!> x(3,pcount), infront(pcount), behind(pcount) (cdata.f90:20-41) and hands
+!> them to the device with qv_upload; qv_velocity then replaces
+!>   ghostp (periodic.f90:8-17) + construct_tree (tree.f90:44-80) + the
+!>   velocity loop of pmotion (timestep.f90:41-47),
+!> qv_ab_step the Adams-Bashforth ladder of pmotion (timestep.f90:48-71),
+!> qv_enforce_periodic the wrap of enforce_periodic (periodic.f90:188-213),
+!> and qv_destroy the empty_tree / deallocate(vtree) of run.f90:132-138.
+!> Default reals of a -fdefault-real-8 build are real(c_double); default
+!> integers are integer(c_int32_t).
+module qv_gpu
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: QV_VEL_BS = 1, QV_VEL_TREE = 2
+  integer(c_int32_t), parameter, public :: QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, &
+                                           QV_BC_OPENX = 2, QV_BC_OPENXY = 3
+  integer(c_int32_t), parameter, public :: QV_FP64 = 0, QV_FP32C = 1
+  integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
+                                       QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
+                                       QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6
+
+  !> run.in parameters of the hot path (cdata.f90:196-416)
+  type, bind(C), public :: qv_params
+    real(c_double) :: quant_circ, corea, box_size, delta, tree_theta, dt
+    integer(c_int32_t) :: velocity, boundary, tree_extra_correction, deriv_order
+    integer(c_int32_t) :: precision, reserved
+  end type
+
+  type, bind(C), public :: qv_timings
+    real(c_float) :: ghost_lia_ms, build_ms, walk_ms, total_ms
+    integer(c_int64_t) :: nodes, evals, visits, leaf_evals, cell_evals, wave_steps
+  end type
+
+  public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
+            qv_velocity, qv_ab_step, qv_enforce_periodic, qv_step, qv_download, &
+            qv_tree_dump, qv_get_timings, qv_status_string, qv_check
+
+  interface
+    integer(c_int) function qv_create(params, ctx) bind(C, name='qv_create')
+      import :: c_int, c_ptr, qv_params
+      type(qv_params), intent(in) :: params
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function qv_destroy(ctx) bind(C, name='qv_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function qv_set_params(ctx, params) bind(C, name='qv_set_params')
+      import :: c_int, c_ptr, qv_params
+      type(c_ptr), value :: ctx
+      type(qv_params), intent(in) :: params
+    end function
+    integer(c_int) function qv_upload(ctx, pcount, x, infront, behind) bind(C, name='qv_upload')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: pcount
+      real(c_double), intent(in) :: x(3, *)
+      integer(c_int32_t), intent(in) :: infront(*), behind(*)
+    end function
+    integer(c_int) function qv_upload_history(ctx, u1, u2, u3) bind(C, name='qv_upload_history')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, u1, u2, u3
+    end function
+    integer(c_int) function qv_velocity(ctx, u, u_s_li, u_s_bs, evals) bind(C, name='qv_velocity')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, u, u_s_li, u_s_bs, evals
+    end function
+    integer(c_int) function qv_ab_step(ctx) bind(C, name='qv_ab_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function qv_enforce_periodic(ctx) bind(C, name='qv_enforce_periodic')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function qv_step(ctx, nsteps) bind(C, name='qv_step')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function qv_download(ctx, x, u, u1, u2, u3) bind(C, name='qv_download')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, u, u1, u2, u3
+    end function
+    integer(c_int) function qv_tree_dump(ctx, out, max_nodes, nnodes) bind(C, name='qv_tree_dump')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, out
+      integer(c_int64_t), value :: max_nodes
+      integer(c_int64_t), intent(out) :: nnodes
+    end function
+    integer(c_int) function qv_get_timings(ctx, t) bind(C, name='qv_get_timings')
+      import :: c_int, c_ptr, qv_timings
+      type(c_ptr), value :: ctx
+      type(qv_timings), intent(out) :: t
+    end function
+    type(c_ptr) function qv_status_string_c(status) bind(C, name='qv_status_string')
+      import :: c_ptr, c_int
+      integer(c_int), value :: status
+    end function
+  end interface
+
+contains
+
+  !> the library's message for a status code
+  function qv_status_string(status) result(msg)
+    integer(c_int), intent(in) :: status
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: n
+    p = qv_status_string_c(status)
+    call c_f_pointer(p, s, [512])
+    n = 0
+    do while (n < 512)
+      if (s(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+  !> stop with the reference's fatal_error layout (cdata.f90:955-970) when a
+  !> call did not return QV_OK
+  subroutine qv_check(status, location)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: location
+    if (status == QV_OK) return
+    write (*,*) '-------------------------FATAL ERROR-------------------------'
+    write (*,*) trim(location), ": ", qv_status_string(status)
+    write (*,*) '-------------------------------------------------------------'
+    stop 1
+  end subroutine
+end module

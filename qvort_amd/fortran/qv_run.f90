@@ -1,0 +1,265 @@
+!> Qvort-style driver with the velocity stage on the GPU.
+!>
+!> Keeps the reference's run.in (label value lines, read as cdata.f90:420-807
+!> does, unknown labels ignored) and its f(:) arrays in the host layout
+!> x(3,pcount), infront, behind (cdata.f90:20-41).  The time loop follows
+!> run.f90:36-169 for the hot path: velocity (ghostp + construct_tree +
+!> calc_velocity), the AB ladder of pmotion, the periodic wrap, t = t + dt,
+!> and every `shots` steps the ts.log line and the var%04d.log snapshot
+!> (output.f90:55-143).  Line operations (pinsert, precon, premove) are not
+!> part of this driver.
+!>
+!> run.in labels used: nsteps shots dt delta box_size velocity boundary
+!> quant_circ corea tree_theta tree_extra_correction deriv_order initf pcount
+!> and gpu_precision (fp64 | fp32c, this driver's own).
+!> initf: single_loop (initial_loop.f90:11-39) or state_file (state.bin:
+!> int32 version, pcount, nsteps, flags, nsample; real64 x(3,pcount);
+!> int32 infront(pcount), behind(pcount); real64 u1, u2, u3 (3,pcount)).
+program qv_run
+  use iso_c_binding
+  use qv_gpu
+  implicit none
+  real(c_double), parameter :: pi = 3.14159265358979324_c_double
+  ! f(:) arrays
+  real(c_double), allocatable, target :: x(:,:), u(:,:), u1(:,:), u2(:,:), u3(:,:)
+  integer(c_int32_t), allocatable, target :: infront(:), behind(:)
+  integer(c_int32_t) :: pcount
+  ! run.in
+  integer :: nsteps = 1, shots = 1, init_pcount = 100
+  real(c_double) :: dt = 1e-6_c_double, delta = 5e-4_c_double, box_size = 0._c_double
+  real(c_double) :: quant_circ = 9.97e-4_c_double, corea = 8.244023e-9_c_double
+  real(c_double) :: tree_theta = 0._c_double
+  logical :: tree_extra_correction = .true.
+  character(len=30) :: velocity = 'Tree', boundary = 'periodic', deriv_order = 'second'
+  character(len=30) :: initf = 'single_loop', gpu_precision = 'fp64'
+  ! state
+  type(qv_params) :: p
+  type(c_ptr) :: ctx
+  type(qv_timings) :: tm
+  integer(c_int64_t), target :: evals
+  integer :: itime, i
+  real(c_double) :: t = 0._c_double
+  logical :: have_hist = .false.
+
+  call read_run_in()
+  call init_filament()
+
+  p%quant_circ = quant_circ
+  p%corea = corea
+  p%box_size = box_size
+  p%delta = delta
+  p%tree_theta = tree_theta
+  p%dt = dt
+  select case (velocity)
+    case ('BS')
+      p%velocity = QV_VEL_BS
+    case ('Tree')
+      p%velocity = QV_VEL_TREE
+    case default
+      call stop_with('init_setup', 'velocity must be BS or Tree on the GPU')
+  end select
+  select case (boundary)
+    case ('periodic')
+      p%boundary = QV_BC_PERIODIC
+    case ('openx')
+      p%boundary = QV_BC_OPENX
+    case ('openxy')
+      p%boundary = QV_BC_OPENXY
+    case ('open')
+      p%boundary = QV_BC_OPEN
+    case default
+      call stop_with('init_setup:', 'incorrect boundary parameter')
+  end select
+  p%tree_extra_correction = merge(1, 0, tree_extra_correction)
+  p%deriv_order = merge(4, 2, deriv_order == 'fourth')
+  p%precision = merge(QV_FP32C, QV_FP64, gpu_precision == 'fp32c')
+  p%reserved = 0
+
+  call qv_check(qv_create(p, ctx), 'qv_create')
+  call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
+  if (have_hist) call qv_check(qv_upload_history(ctx, c_loc(u1), c_loc(u2), c_loc(u3)), &
+                               'qv_upload_history')
+  call execute_command_line('mkdir -p data')
+  write(*,'(a,i0,a,a,a,a)') ' qv_run: ', count(infront > 0), ' points, velocity ', &
+        trim(velocity), ', precision ', trim(gpu_precision)
+
+  do itime = 1, nsteps
+    call qv_check(qv_velocity(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(evals)), &
+                  'qv_velocity')
+    call qv_check(qv_ab_step(ctx), 'qv_ab_step')
+    call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
+    t = t + dt
+    if (mod(itime, shots) == 0) then
+      call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
+                    'qv_download')
+      call print_info(itime / shots)
+      call printf(itime / shots)
+    end if
+  end do
+  call qv_check(qv_get_timings(ctx, tm), 'qv_get_timings')
+  write(*,'(a,f10.3,a,f10.3,a,f10.3,a)') ' last stage: build ', tm%build_ms, ' ms, walk ', &
+        tm%walk_ms, ' ms, total ', tm%total_ms, ' ms'
+  call qv_check(qv_destroy(ctx), 'qv_destroy')
+
+contains
+
+  subroutine stop_with(location, message)
+    character(len=*), intent(in) :: location, message
+    write (*,*) '-------------------------FATAL ERROR-------------------------'
+    write (*,*) trim(location), ": ", trim(message)
+    write (*,*) '-------------------------------------------------------------'
+    stop 1
+  end subroutine
+
+  !> label value lines; the first blank ends the label (cdata.f90:437-441)
+  subroutine read_run_in()
+    character(len=100) :: buffer, label
+    integer :: pos, ios, fh
+    open(newunit=fh, file='run.in', status='old', action='read', iostat=ios)
+    if (ios /= 0) call stop_with('read_run_in', 'run.in not found')
+    do
+      read(fh, '(A)', iostat=ios) buffer
+      if (ios /= 0) exit
+      pos = scan(buffer, ' ')
+      label = buffer(1:pos)
+      buffer = buffer(pos+1:)
+      select case (label)
+        case ('nsteps')
+          read(buffer, *, iostat=ios) nsteps
+        case ('shots')
+          read(buffer, *, iostat=ios) shots
+        case ('dt')
+          read(buffer, *, iostat=ios) dt
+        case ('delta')
+          read(buffer, *, iostat=ios) delta
+        case ('box_size')
+          read(buffer, *, iostat=ios) box_size
+        case ('velocity')
+          read(buffer, *, iostat=ios) velocity
+        case ('boundary')
+          read(buffer, *, iostat=ios) boundary
+        case ('quant_circ')
+          read(buffer, *, iostat=ios) quant_circ
+        case ('corea')
+          read(buffer, *, iostat=ios) corea
+        case ('tree_theta')
+          read(buffer, *, iostat=ios) tree_theta
+        case ('tree_extra_correction')
+          read(buffer, *, iostat=ios) tree_extra_correction
+        case ('deriv_order')
+          read(buffer, *, iostat=ios) deriv_order
+        case ('initf')
+          read(buffer, *, iostat=ios) initf
+        case ('pcount')
+          read(buffer, *, iostat=ios) init_pcount
+        case ('gpu_precision')
+          read(buffer, *, iostat=ios) gpu_precision
+      end select
+      ios = 0
+    end do
+    close(fh)
+    if (.not. box_size > 0._c_double) call stop_with('init_setup:', 'box size is less than zero')
+    if (velocity == 'Tree' .and. .not. tree_theta > 0._c_double) &
+      call stop_with('init_setup', 'tree_theta must be > 0 for the Tree velocity')
+  end subroutine
+
+  subroutine init_filament()
+    integer :: fh, version, nst, flags, nsample
+    select case (initf)
+      case ('single_loop')
+        ! setup_single_loop, initial_loop.f90:11-39
+        pcount = init_pcount
+        call alloc_f()
+        block
+          real(c_double) :: radius
+          radius = (0.75_c_double * pcount * delta) / (2 * pi)
+          do i = 1, pcount
+            x(1, i) = radius * sin(pi * real(2 * i - 1, c_double) / pcount)
+            x(2, i) = radius * cos(pi * real(2 * i - 1, c_double) / pcount)
+            x(3, i) = 0._c_double
+            infront(i) = i + 1
+            behind(i) = i - 1
+          end do
+          infront(pcount) = 1
+          behind(1) = pcount
+        end block
+      case ('state_file')
+        open(newunit=fh, file='state.bin', access='stream', form='unformatted', status='old', &
+             action='read')
+        read(fh) version, pcount, nst, flags, nsample
+        call alloc_f()
+        read(fh) x
+        read(fh) infront
+        read(fh) behind
+        read(fh) u1, u2, u3
+        close(fh)
+        have_hist = any(u1 /= 0._c_double) .or. any(u2 /= 0._c_double) .or. &
+                    any(u3 /= 0._c_double)
+      case default
+        call stop_with('init_setup', 'initf must be single_loop or state_file')
+    end select
+  end subroutine
+
+  subroutine alloc_f()
+    allocate(x(3, pcount), u(3, pcount), u1(3, pcount), u2(3, pcount), u3(3, pcount))
+    allocate(infront(pcount), behind(pcount))
+    x = 0._c_double; u = 0._c_double; u1 = 0._c_double; u2 = 0._c_double; u3 = 0._c_double
+    infront = 0; behind = 0
+  end subroutine
+
+  !> the ts.log line of print_info (output.f90:55-77): columns we do not
+  !> compute here (recon count, maxdu, curvature, removals) are 0
+  subroutine print_info(filenumber)
+    integer, intent(in) :: filenumber
+    integer :: fh, nact, j, k
+    real(c_double) :: length, maxu, d(3)
+    nact = count(infront > 0)
+    length = 0._c_double
+    maxu = 0._c_double
+    do j = 1, pcount
+      if (infront(j) == 0) cycle
+      d = x(:, infront(j)) - x(:, j)
+      if (boundary /= 'open') then
+        do k = 1, 3
+          if (boundary == 'openx' .and. k == 1) cycle
+          if (boundary == 'openxy' .and. k < 3) cycle
+          if (d(k) > box_size / 2) d(k) = d(k) - box_size
+          if (d(k) < -box_size / 2) d(k) = d(k) + box_size
+        end do
+      end if
+      length = length + sqrt(sum(d**2))
+      maxu = max(maxu, sqrt(sum(u(:, j)**2)))
+    end do
+    open(newunit=fh, file='data/ts.log', position='append')
+    if (filenumber == 1) write(fh, *) '%--var--------t-------pcount-------recon----avg_d-----length&
+        &--------maxu---------maxdu-----num eval----curv------removed'
+    write(fh, '(i6.4,f13.7,i10.1,i13.1,f7.4,f13.6,f13.5,f13.5,f10.2,f10.2,i13.1)') &
+      filenumber, t, nact, 0, length / nact / delta, length, maxu, 0._c_double, &
+      real(evals, c_double) / nact, 0._c_double, 0
+    close(fh)
+  end subroutine
+
+  !> data/var%04d.log as printf writes it (output.f90:116-143); u_mf, curv,
+  !> stretch and f_mf are not computed here and written as 0
+  subroutine printf(filenumber)
+    integer, intent(in) :: filenumber
+    character(len=40) :: print_file
+    integer :: fh
+    real(c_double), allocatable :: z(:)
+    allocate(z(pcount))
+    z = 0._c_double
+    write(print_file, '(a,i4.4,a)') './data/var', filenumber, '.log'
+    open(newunit=fh, file=print_file, status='replace', form='unformatted', access='stream')
+    write(fh) t
+    write(fh) pcount
+    write(fh) x(1, :)
+    write(fh) x(2, :)
+    write(fh) x(3, :)
+    write(fh) infront
+    write(fh) u(1, :)
+    write(fh) u(2, :)
+    write(fh) u(3, :)
+    write(fh) z, z, z, z, z, z, z, z
+    close(fh)
+  end subroutine
+end program

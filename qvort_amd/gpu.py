@@ -45,7 +45,8 @@ class QvParams(C.Structure):
 class QvTimings(C.Structure):
     _fields_ = [("ghost_lia_ms", C.c_float), ("build_ms", C.c_float), ("walk_ms", C.c_float),
                 ("total_ms", C.c_float), ("nodes", C.c_int64), ("evals", C.c_int64),
-                ("visits", C.c_int64), ("leaf_evals", C.c_int64), ("cell_evals", C.c_int64)]
+                ("visits", C.c_int64), ("leaf_evals", C.c_int64), ("cell_evals", C.c_int64),
+                ("wave_steps", C.c_int64)]
 
 
 class QvortError(RuntimeError):
