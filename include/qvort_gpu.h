@@ -126,6 +126,30 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2,
  * are written (out may be NULL to query the count). */
 int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes);
 
+/* ---- derivatives of the direct-sum velocity (velocity = QV_VEL_BS) ----
+ * No counterpart in the reference: the adjoint, for gradients of objectives
+ * (line length, energy) through many steps, and the tangent map, for
+ * implicit solvers.  U(x) is the velocity of qv_velocity (LIA term +
+ * biot_savart + biot_savart_shift) at the context's positions; the loops
+ * must be consistent (infront(behind(i)) == i), else QV_ERR_ARG.
+ * precision QV_FP64: FP64 pair terms; QV_FP32C: FP32 pair terms with FP64
+ * relative vectors and sums.  Arrays in the layout of x(3,pcount). */
+/* xbar = (dU/dx)^T w */
+int qv_vjp(qv_ctx *ctx, const double *w, double *xbar);
+/* jv = (dU/dx) v */
+int qv_jvp(qv_ctx *ctx, const double *v, double *jv);
+
+/* ---- device-pointer variants for callers that keep the state in device
+ * memory (e.g. a PyTorch harness).  No counterpart in the reference.  All
+ * pointers are device buffers of (pcount,3) doubles; d_x (may be NULL to
+ * keep the context's positions) replaces the context's positions. */
+/* launch every later call on `hip_stream` (a hipStream_t, used as given:
+ * NULL is HIP's null stream); a context starts on a stream of its own */
+int qv_set_stream(qv_ctx *ctx, void *hip_stream);
+int qv_velocity_device(qv_ctx *ctx, const double *d_x, double *d_u);
+int qv_vjp_device(qv_ctx *ctx, const double *d_x, const double *d_w, double *d_xbar);
+int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_jv);
+
 int qv_get_timings(qv_ctx *ctx, qv_timings *t);
 const char *qv_status_string(int status);
 

@@ -203,6 +203,53 @@ void dump_rec(DumpCtx &D, int64_t n) {
   }
 }
 
+int adj_reserve(Ctx &c) {
+  if (c.adj_cap >= c.pcount) return QV_OK;
+  TRY(dalloc(c.adj, (size_t)24 * c.pcount));
+  TRY(valloc(c.aw, c.pcount));
+  TRY(valloc(c.ar, c.pcount));
+  c.adj_cap = c.pcount;
+  return QV_OK;
+}
+
+// derivative of the direct-sum velocity at the device positions; `v` and
+// the result are AoS (pcount,3) device buffers
+int derivative(Ctx &c, const double *d_in, double *d_out, bool vjp) {
+  if (c.pcount == 0) return QV_ERR_STATE;
+  if (c.p.velocity != QV_VEL_BS || !c.links_consistent) return QV_ERR_ARG;
+  int rc = adj_reserve(c);
+  if (rc) return rc;
+  TRY(launch_ghost_lia(c));
+  TRY(launch_aos_to_soa(c, d_in, c.aw));
+  if (vjp) TRY(launch_vjp(c, c.aw, c.ar, c.p.precision == QV_FP32C ? 1 : 0));
+  else TRY(launch_jvp(c, c.aw, c.ar));
+  TRY(launch_soa_to_aos(c, c.ar, d_out));
+  return QV_OK;
+}
+
+int host_derivative(qv_ctx *ctx, const double *in, double *out, bool vjp) {
+  if (!ctx || !in || !out) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  int rc = adj_reserve(*c);
+  if (rc) return rc;
+  // the staging buffer carries the input; the result comes back through it
+  TRY(hipMemcpyAsync(c->stage, in, (size_t)3 * c->pcount * sizeof(double),
+                     hipMemcpyHostToDevice, c->stream));
+  rc = derivative(*c, c->stage, c->stage, vjp);
+  if (rc) return rc;
+  TRY(hipMemcpyAsync(out, c->stage, (size_t)3 * c->pcount * sizeof(double),
+                     hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int set_positions_device(Ctx &c, const double *d_x) {
+  if (!d_x) return QV_OK;
+  TRY(launch_aos_to_soa(c, d_x, c.x));
+  return QV_OK;
+}
+
 }  // namespace
 }  // namespace qv
 
@@ -221,10 +268,11 @@ int qv_create(const qv_params *params, qv_ctx **out) {
   if (!c) return QV_ERR_HIP;
   c->p = *params;
   if (hipGetDevice(&c->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return QV_ERR_HIP;
   }
+  c->stream = c->own_stream;
   for (auto &e : c->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       delete c;
@@ -247,12 +295,13 @@ int qv_destroy(qv_ctx *ctx) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream && c->own_stream != c->stream) (void)hipStreamSynchronize(c->own_stream);
   Vec3 *vs[] = {&c->x, &c->gi, &c->gb, &c->gii, &c->gbb, &c->u, &c->uli, &c->ubs,
-                &c->u1, &c->u2, &c->u3, &c->xs, &c->gs};
+                &c->u1, &c->u2, &c->u3, &c->xs, &c->gs, &c->aw, &c->ar};
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
                 c->slot_s, c->nnode_of, c->node_off, c->sort_tmp, c->counters, c->evals,
-                c->big_buf, c->partial,
+                c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner};
   for (void *p : ps) (void)hipFree(p);
@@ -260,7 +309,7 @@ int qv_destroy(qv_ctx *ctx) {
   if (c->h_evals) (void)hipHostFree(c->h_evals);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return QV_OK;
 }
@@ -288,12 +337,17 @@ int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infro
     if (infront[a - 1] == 0 || infront[b - 1] == 0) return QV_ERR_ARG;
     ++nact;
   }
+  bool consistent = true;
+  for (int32_t i = 0; i < pcount && consistent; ++i)
+    if (infront[i] != 0 && (behind[infront[i] - 1] != i + 1 || infront[behind[i] - 1] != i + 1))
+      consistent = false;
   if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
   int rc = reserve(*c, pcount);
   if (rc) return rc;
   c->pcount = pcount;
   c->nact = nact;
   c->have_velocity = false;
+  c->links_consistent = consistent;
   TRY(hipMemcpyAsync(c->infront, infront, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
                      c->stream));
   TRY(hipMemcpyAsync(c->behind, behind, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -410,6 +464,55 @@ int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes) {
   const int64_t k = D.k;
   if (nnodes) *nnodes = k;
   return QV_OK;
+}
+
+int qv_vjp(qv_ctx *ctx, const double *w, double *xbar) {
+  return host_derivative(ctx, w, xbar, true);
+}
+
+int qv_jvp(qv_ctx *ctx, const double *v, double *jv) {
+  return host_derivative(ctx, v, jv, false);
+}
+
+int qv_set_stream(qv_ctx *ctx, void *stream) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  // used as given: NULL is HIP's null stream, which orders with the
+  // caller's other blocking streams
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (s != c->stream) TRY(hipStreamSynchronize(c->stream));
+  c->stream = s;
+  return QV_OK;
+}
+
+int qv_velocity_device(qv_ctx *ctx, const double *d_x, double *d_u) {
+  if (!ctx || !d_u) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  int rc = set_positions_device(*c, d_x);
+  if (rc) return rc;
+  rc = velocity_stage(*c);
+  if (rc) return rc;
+  TRY(launch_soa_to_aos(*c, c->u, d_u));
+  return QV_OK;
+}
+
+int qv_vjp_device(qv_ctx *ctx, const double *d_x, const double *d_w, double *d_xbar) {
+  if (!ctx || !d_w || !d_xbar) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  int rc = set_positions_device(*c, d_x);
+  if (rc) return rc;
+  return derivative(*c, d_w, d_xbar, true);
+}
+
+int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_jv) {
+  if (!ctx || !d_v || !d_jv) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  int rc = set_positions_device(*c, d_x);
+  if (rc) return rc;
+  return derivative(*c, d_v, d_jv, false);
 }
 
 int qv_get_timings(qv_ctx *ctx, qv_timings *t) {

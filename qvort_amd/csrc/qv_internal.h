@@ -70,7 +70,8 @@ struct Tree {
 struct Ctx {
   qv_params p{};
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream of every launch (own or the caller's)
+  hipStream_t own_stream = nullptr;
   int32_t pcount = 0, cap = 0;
   int32_t nact = 0;
   bool have_velocity = false;
@@ -96,6 +97,11 @@ struct Ctx {
   int32_t *h_counters = nullptr;
   int64_t *h_evals = nullptr;
   qv_timings timings{};
+  // adjoint: 24 doubles of scratch per slot, tangent/cotangent and result
+  double *adj = nullptr;
+  Vec3 aw, ar;
+  int32_t adj_cap = 0;
+  bool links_consistent = false;  // infront(behind(i)) == i for every point
   hipEvent_t ev[6] = {};
 };
 
@@ -143,5 +149,7 @@ hipError_t launch_enforce_periodic(Ctx &c);
 hipError_t launch_direct(Ctx &c);
 int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
+hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
+hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 
 }  // namespace qv

@@ -5,6 +5,7 @@
 // without contraction, so these kernels reproduce the flang build of the
 // reference bit for bit (log() aside, which may differ in the last ulp).
 #include "qv_internal.h"
+#include "qv_lia.h"
 
 #pragma clang fp contract(off)
 
@@ -12,14 +13,6 @@ namespace qv {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ inline double sq(double a) { return a * a; }
-
-// dist_gen, derivatives.f90:79-86
-__device__ inline double dist_gen(double ax, double ay, double az, double bx,
-                                  double by, double bz) {
-  return sqrt((sq(ax - bx) + sq(ay - by)) + sq(az - bz));
-}
 
 __global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict__ x,
                              double *__restrict__ y, double *__restrict__ z, int n) {
@@ -52,51 +45,6 @@ struct Lia {
   int order;
   bool px, py, pz;  // periodic axes
 };
-
-// get_deriv_1 / get_deriv_2, derivatives.f90:11-75
-__device__ void derivs(const Lia &P, const double x[3], const double gi[3],
-                       const double gb[3], const double gii[3], const double gbb[3],
-                       double s1[3], double s2[3]) {
-  const double di = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
-  const double db = dist_gen(x[0], x[1], x[2], gb[0], gb[1], gb[2]);
-  if (P.order != 4) {
-    const double den = (2. * db) * di;
-    for (int k = 0; k < 3; ++k) s1[k] = ((db * gi[k] + (di - db) * x[k]) - di * gb[k]) / den;
-    for (int k = 0; k < 3; ++k)
-      s2[k] = 2. * ((gi[k] / (di * (di + db)) - x[k] / (di * db)) + gb[k] / (db * (di + db)));
-    return;
-  }
-  const double dii = dist_gen(gii[0], gii[1], gii[2], gi[0], gi[1], gi[2]);
-  const double dbb = dist_gen(gbb[0], gbb[1], gbb[2], gb[0], gb[1], gb[2]);
-  {
-    double A = db * (di * di) + (db * di) * dii;
-    A = A / (((dbb * (dbb + db)) * ((dbb + db) + di)) * (((dbb + db) + di) + dii));
-    double B = ((-(dbb * (di * di)) - db * (di * di)) - (dbb * di) * dii) - (db * di) * dii;
-    B = B / (((dbb * db) * (db + di)) * ((db + di) + dii));
-    double D = (((dbb * db) * di + (db * db) * di) + (dbb * db) * dii) + (db * db) * dii;
-    D = D / (((di * dii) * (db + di)) * ((dbb + db) + di));
-    double E = -(di * (db * db)) - (dbb * db) * di;
-    E = E / (((dii * (di + dii)) * ((db + di) + dii)) * (((dbb + db) + di) + dii));
-    const double C = -(((A + B) + D) + E);
-    for (int k = 0; k < 3; ++k)
-      s1[k] = (((A * gbb[k] + B * gb[k]) + C * x[k]) + D * gi[k]) + E * gii[k];
-  }
-  {
-    double A = 2. * (((-((2. * db) * di) + di * di) - db * dii) + di * dii);
-    A = A / (((dbb * (dbb + db)) * ((dbb + db) + di)) * (((dbb + db) + di) + dii));
-    double B = 2. * ((((((2. * dbb) * di + (2. * db) * di) - di * di) + dbb * dii) + db * dii) -
-                     di * dii);
-    B = B / (((dbb * db) * (db + di)) * ((db + di) + dii));
-    double D = 2. * (((((-(dbb * db) - db * db) + dbb * di) + (2. * db) * di) + dbb * dii) +
-                     (2. * db) * dii);
-    D = D / (((di * dii) * (db + di)) * ((dbb + db) + di));
-    double E = 2. * (((dbb * db + db * db) - dbb * di) - (2. * db) * di);
-    E = E / (((dii * (di + dii)) * ((db + di) + dii)) * (((dbb + db) + di) + dii));
-    const double C = -(((A + B) + D) + E);
-    for (int k = 0; k < 3; ++k)
-      s2[k] = (((A * gbb[k] + B * gb[k]) + C * x[k]) + D * gi[k]) + E * gii[k];
-  }
-}
 
 // ghostp (periodic.f90:8-184) fused with the local term of calc_velocity
 // (timestep.f90:221-225 / 270-274): u_LI = Gamma/(4 pi) ln((1.21/a)
@@ -139,14 +87,11 @@ k_ghost_lia(Lia P, int n, const double *__restrict__ X, const double *__restrict
   gbx[i] = gb[0]; gby[i] = gb[1]; gbz[i] = gb[2];
   giix[i] = gii[0]; giiy[i] = gii[1]; giiz[i] = gii[2];
   gbbx[i] = gbb[0]; gbby[i] = gbb[1]; gbbz[i] = gbb[2];
-  double s1[3], s2[3];
-  derivs(P, x, gi, gb, gii, gbb, s1, s2);
-  const double li = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
-  const double lb = dist_gen(x[0], x[1], x[2], gb[0], gb[1], gb[2]);
-  const double beta = (P.qc / (4. * kPi)) * log((1.21 / P.corea) * sqrt(li * lb));
-  ulx[i] = beta * (s1[1] * s2[2] - s1[2] * s2[1]);
-  uly[i] = beta * (s1[2] * s2[0] - s1[0] * s2[2]);
-  ulz[i] = beta * (s1[0] * s2[1] - s1[1] * s2[0]);
+  double ul[3];
+  lia_term<double>(P.qc, P.corea, P.order, x, gi, gb, gii, gbb, ul);
+  ulx[i] = ul[0];
+  uly[i] = ul[1];
+  ulz[i] = ul[2];
 }
 
 __device__ inline double maxabs3(double a, double b, double c) {

@@ -77,20 +77,21 @@ __device__ inline bool accept_full(double d2, double w, double g, double theta, 
 // leaf term, tree.f90:380-392, in the reference's operation order up to the
 // factor Gamma/(4 pi), which is applied once to the target's sum.  Added to u
 // only where `on` (the other lanes may hold inf/nan terms).
+template <bool kShift>
 __device__ inline void leaf64(bool on, const double xi[3], double cx, double cy, double cz,
                               double gx, double gy, double gz, const double s[3], double u[3]) {
-  const double R1[3] = {(cx + s[0]) - xi[0], (cy + s[1]) - xi[1], (cz + s[2]) - xi[2]};
-  const double R2[3] = {(gx + s[0]) - xi[0], (gy + s[1]) - xi[1], (gz + s[2]) - xi[2]};
+  // x + shift with shift = 0 is x itself, so the central shell skips the add
+  const double R1[3] = {(kShift ? cx + s[0] : cx) - xi[0], (kShift ? cy + s[1] : cy) - xi[1],
+                        (kShift ? cz + s[2] : cz) - xi[2]};
+  const double R2[3] = {(kShift ? gx + s[0] : gx) - xi[0], (kShift ? gy + s[1] : gy) - xi[1],
+                        (kShift ? gz + s[2] : gz) - xi[2]};
   const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
   const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
   const double c = (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
-  const double f = (a + b) / c;
-  const double w0 = f * (R1[1] * R2[2] - R1[2] * R2[1]);
-  const double w1 = f * (R1[2] * R2[0] - R1[0] * R2[2]);
-  const double w2 = f * (R1[0] * R2[1] - R1[1] * R2[0]);
-  u[0] = u[0] + (on ? w0 : 0.);
-  u[1] = u[1] + (on ? w1 : 0.);
-  u[2] = u[2] + (on ? w2 : 0.);
+  const double f = on ? (a + b) / c : 0.;  // finite cross product, so 0 * it = 0
+  u[0] = u[0] + f * (R1[1] * R2[2] - R1[2] * R2[1]);
+  u[1] = u[1] + f * (R1[2] * R2[0] - R1[0] * R2[2]);
+  u[2] = u[2] + f * (R1[0] * R2[1] - R1[1] * R2[0]);
 }
 
 // cell term, tree.f90:400-411 (without Gamma/(4 pi)); r = x_i - (cent +
@@ -100,7 +101,7 @@ __device__ inline void cell64(bool on, const double r[3], double d2,
   const double d = sqrt(d2);
   const double d3 = (d * d) * d;
   const double d5 = (((d * d) * d) * d) * d;
-  const double id3 = 1. / d3, id5 = 1. / d5;
+  const double id3 = on ? 1. / d3 : 0., id5 = on ? 1. / d5 : 0.;
   const double c0 = m[0], c1 = m[1], c2 = m[2];
   const double *C = m + 3;  // C[3*i + k] = Cij(i,k)
   // eps(:,k,l) C(k,l): antisymmetric part
@@ -109,12 +110,9 @@ __device__ inline void cell64(bool on, const double r[3], double d2,
   const double v0 = (r[0] * C[0] + r[1] * C[3]) + r[2] * C[6];
   const double v1 = (r[0] * C[1] + r[1] * C[4]) + r[2] * C[7];
   const double v2 = (r[0] * C[2] + r[1] * C[5]) + r[2] * C[8];
-  const double t0 = ((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5;
-  const double t1 = ((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5;
-  const double t2 = ((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5;
-  u[0] = u[0] + (on ? t0 : 0.);
-  u[1] = u[1] + (on ? t1 : 0.);
-  u[2] = u[2] + (on ? t2 : 0.);
+  u[0] = u[0] + (((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5);
+  u[1] = u[1] + (((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5);
+  u[2] = u[2] + (((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5);
 }
 
 // FP32 terms.  The parity code of this file is compiled without
@@ -147,10 +145,12 @@ __device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, doubl
     const float cc = fmaf_(c2, c2, fmaf_(c1, c1, c0 * c0));
     f = g * (ab - dt) * __builtin_amdgcn_rcpf(cc);
   }
-  const double fd = on ? (double)f : 0.;
-  acc.s0 = fma(fd, (double)(on ? c0 : 0.f), acc.s0);
-  acc.s1 = fma(fd, (double)(on ? c1 : 0.f), acc.s1);
-  acc.s2 = fma(fd, (double)(on ? c2 : 0.f), acc.s2);
+  // lanes that are off may hold inf/nan in f (e.g. their own point); R1 x e
+  // is finite, so selecting f alone zeroes their contribution
+  const double fd = (double)(on ? f : 0.f);
+  acc.s0 = fma(fd, (double)c0, acc.s0);
+  acc.s1 = fma(fd, (double)c1, acc.s1);
+  acc.s2 = fma(fd, (double)c2, acc.s2);
 }
 
 // FP32 cell: m6 = circ(3), eps:C(3) from the node record, C(9) from mp32.
@@ -159,10 +159,11 @@ __device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, doubl
 __device__ inline void cell32(bool on, const double rd[3], double d2, const float *m6,
                               const float *__restrict__ C, AccF64 &acc) {
   const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
-  const float id = __builtin_amdgcn_rsqf((float)d2);
+  // lanes that are off use d2 = 1 (finite terms) and weight 0
+  const float id = __builtin_amdgcn_rsqf(on ? (float)d2 : 1.f);
   const float id2 = id * id;
   const float q = 3.f * id2;
-  const double id3 = on ? (double)(id2 * id) : 0.;
+  const double id3 = (double)(on ? id2 * id : 0.f);
   const float v0 = fmaf_(r2, C[6], fmaf_(r1, C[3], r0 * C[0]));
   const float v1 = fmaf_(r2, C[7], fmaf_(r1, C[4], r0 * C[1]));
   const float v2 = fmaf_(r2, C[8], fmaf_(r1, C[5], r0 * C[2]));
@@ -172,9 +173,9 @@ __device__ inline void cell32(bool on, const double rd[3], double d2, const floa
   const float Y0 = fmaf_(r1, v2, -r2 * v1);
   const float Y1 = fmaf_(r2, v0, -r0 * v2);
   const float Y2 = fmaf_(r0, v1, -r1 * v0);
-  acc.s0 = fma((double)(on ? fmaf_(Y0, q, X0) : 0.f), id3, acc.s0);
-  acc.s1 = fma((double)(on ? fmaf_(Y1, q, X1) : 0.f), id3, acc.s1);
-  acc.s2 = fma((double)(on ? fmaf_(Y2, q, X2) : 0.f), id3, acc.s2);
+  acc.s0 = fma((double)fmaf_(Y0, q, X0), id3, acc.s0);
+  acc.s1 = fma((double)fmaf_(Y1, q, X1), id3, acc.s1);
+  acc.s2 = fma((double)fmaf_(Y2, q, X2), id3, acc.s2);
 }
 
 struct alignas(16) StackEntry {
@@ -182,7 +183,92 @@ struct alignas(16) StackEntry {
   uint64_t mask;
 };
 
-template <int kMode, class Acc>
+// per-wavefront walk state
+struct WalkCounts {
+  uint32_t nvis = 0, nleaf = 0, ncell = 0, nstep = 0;
+  bool singular = false;
+};
+
+// one shell's walk of the whole tree for the wavefront's lanes (tree_walk
+// called with one shift, timestep.f90:277 / :285)
+template <int kMode, bool kShift>
+__device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const double xi[3],
+                                  int t, int bi, int fi, uint64_t active, int lane,
+                                  StackEntry *st, double u[3], AccF64 &acc, WalkCounts &wc) {
+  // current entry (node, end of its sibling list, lanes) in SGPRs; the LDS
+  // stack holds only sibling lists left behind when descending
+  int n = 0, e = (int)A.nnodes;
+  uint64_t M = active;
+  int sp = 0;
+  while (true) {
+    const NodeRec rec = A.rec[n];
+    ++wc.nstep;
+    const int skip = uni(rec.skip), j = uni(rec.j);
+    const bool mine = (M >> lane) & 1ull;
+    wc.nvis += (uint32_t)__popcll(M);
+    // r = x_i - (cent + shift), d2 as the reference forms dist (tree.f90:351-353)
+    const double cs0 = kShift ? rec.cx + sv[0] : rec.cx;
+    const double cs1 = kShift ? rec.cy + sv[1] : rec.cy;
+    const double cs2 = kShift ? rec.cz + sv[2] : rec.cz;
+    const double r[3] = {xi[0] - cs0, xi[1] - cs1, xi[2] - cs2};
+    const double d2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    uint64_t O = 0;
+    if (j >= 0) {
+      // leaf: the single point's segment unless it is i, behind(i), infront(i)
+      const bool on = mine && j != t && j != bi && j != fi;
+      const uint64_t L = __ballot(on);
+      if (L) {
+        if (__ballot(on && d2 < 1e-30)) {
+          if (on && d2 < 1e-30 && sqrt(d2) < 2.220446049250313e-16) wc.singular = true;
+        }
+        if (kMode == QV_FP64) {
+          leaf64<kShift>(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
+        } else {
+          leaf32(on, -r[0], -r[1], -r[2], rec.g[0] - rec.cx, rec.g[1] - rec.cy,
+                 rec.g[2] - rec.cz, acc);
+        }
+        wc.nleaf += (uint32_t)__popcll(L);
+      }
+    } else {
+      bool ok;
+      if (rec.T >= 0.) {
+        ok = mine && d2 >= rec.T;
+      } else {
+        const double2 g = A.geo[n];
+        ok = mine && accept_full(d2, g.x, g.y, A.theta, A.extra);
+      }
+      const uint64_t Acc_ = __ballot(ok);
+      if (Acc_) {
+        if (kMode == QV_FP64) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
+        else cell32(ok, r, d2, rec.m6, A.mp32 + 16 * (int64_t)n, acc);
+        wc.ncell += (uint32_t)__popcll(Acc_);
+      }
+      O = M & ~Acc_;
+    }
+    if (O != 0 && n + 1 != skip) {
+      // descend into the children [n+1, skip) with the opening lanes
+      if (skip != e) {
+        st[sp].node = skip;
+        st[sp].end = e;
+        st[sp].mask = M;
+        ++sp;
+      }
+      e = skip;
+      n = n + 1;
+      M = O;
+    } else if (skip != e) {
+      n = skip;  // next sibling, same lanes
+    } else {
+      if (sp == 0) break;
+      --sp;
+      n = uni(st[sp].node);
+      e = uni(st[sp].end);
+      M = uni64(st[sp].mask);
+    }
+  }
+}
+
+template <int kMode>
 __global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
   // XCD-aware order: consecutive blocks of the target list go to one XCD
@@ -204,114 +290,39 @@ __global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
   }
   const uint64_t active = __ballot(valid);
   if (active == 0) return;
-  // one running sum: the central shell's is banked in uc after shell 0, the
-  // image shells then accumulate from zero (timestep.f90:276-288)
-  double uc[3] = {0., 0., 0.};
-  double u[3] = {0., 0., 0.};
-  Acc acc;
-  // per-wavefront counts (scalar): lane visits, leaf and cell interactions
-  uint32_t nvis = 0, nleaf = 0, ncell = 0, nstep = 0;
-  bool singular = false;
+  WalkCounts wc;
   StackEntry *st = stack[wid];
-  for (int sh = 0; sh < S.n; ++sh) {
+  // central shell, then the image shells into a second sum
+  // (timestep.f90:276-288)
+  double uc[3] = {0., 0., 0.};
+  {
+    const double z[3] = {0., 0., 0.};
+    double u[3] = {0., 0., 0.};
+    AccF64 acc;
+    walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, lane, st, u, acc, wc);
+    if (kMode == QV_FP64) { uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2]; }
+    else acc.get(uc[0], uc[1], uc[2]);
+  }
+  double u[3] = {0., 0., 0.};
+  AccF64 acc;
+  for (int sh = 1; sh < S.n; ++sh) {
     const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    if (sh == 1) {
-      if (kMode == QV_FP64) {
-        uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2];
-        u[0] = u[1] = u[2] = 0.;
-      } else {
-        acc.get(uc[0], uc[1], uc[2]);
-        acc = Acc();
-      }
-    }
-    // current entry (node, end of its sibling list, lanes) in SGPRs; the LDS
-    // stack holds only sibling lists left behind when descending
-    int n = 0, e = (int)A.nnodes;
-    uint64_t M = active;
-    int sp = 0;
-    while (true) {
-      const NodeRec rec = A.rec[n];
-      ++nstep;
-      const int skip = uni(rec.skip), j = uni(rec.j);
-      const bool mine = (M >> lane) & 1ull;
-      nvis += (uint32_t)__popcll(M);
-      const double cs0 = rec.cx + sv[0], cs1 = rec.cy + sv[1], cs2 = rec.cz + sv[2];
-      const double r[3] = {xi[0] - cs0, xi[1] - cs1, xi[2] - cs2};
-      const double d2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
-      uint64_t O = 0;
-      if (j >= 0) {
-        // leaf: the single point's segment unless it is i, behind(i), infront(i)
-        const bool on = mine && j != t && j != bi && j != fi;
-        const uint64_t L = __ballot(on);
-        if (L) {
-          if (__ballot(on && d2 < 1e-30)) {
-            if (on && d2 < 1e-30 && sqrt(d2) < 2.220446049250313e-16) singular = true;
-          }
-          if (kMode == QV_FP64) {
-            leaf64(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
-          } else {
-            leaf32(on, -r[0], -r[1], -r[2], rec.g[0] - rec.cx, rec.g[1] - rec.cy,
-                   rec.g[2] - rec.cz, acc);
-          }
-          nleaf += (uint32_t)__popcll(L);
-        }
-      } else {
-        bool ok;
-        if (rec.T >= 0.) {
-          ok = mine && d2 >= rec.T;
-        } else {
-          const double2 g = A.geo[n];
-          ok = mine && accept_full(d2, g.x, g.y, A.theta, A.extra);
-        }
-        const uint64_t Acc_ = __ballot(ok);
-        if (Acc_) {
-          if (kMode == QV_FP64) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
-          else cell32(ok, r, d2, rec.m6, A.mp32 + 16 * (int64_t)n, acc);
-          ncell += (uint32_t)__popcll(Acc_);
-        }
-        O = M & ~Acc_;
-      }
-      if (O != 0 && n + 1 != skip) {
-        // descend into the children [n+1, skip) with the opening lanes
-        if (skip != e) {
-          st[sp].node = skip;
-          st[sp].end = e;
-          st[sp].mask = M;
-          ++sp;
-        }
-        e = skip;
-        n = n + 1;
-        M = O;
-      } else if (skip != e) {
-        n = skip;  // next sibling, same lanes
-      } else {
-        if (sp == 0) break;
-        --sp;
-        n = uni(st[sp].node);
-        e = uni(st[sp].end);
-        M = uni64(st[sp].mask);
-      }
-    }
+    walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, lane, st, u, acc, wc);
   }
   // interaction, visit, leaf and step counts (64-bit) and the error flag
   if (lane == 0) {
-    atomicAdd(A.evals + 0, (unsigned long long)nleaf + ncell);
-    atomicAdd(A.evals + 1, (unsigned long long)nvis);
-    atomicAdd(A.evals + 2, (unsigned long long)nleaf);
-    atomicAdd(A.evals + 3, (unsigned long long)nstep);
+    atomicAdd(A.evals + 0, (unsigned long long)wc.nleaf + wc.ncell);
+    atomicAdd(A.evals + 1, (unsigned long long)wc.nvis);
+    atomicAdd(A.evals + 2, (unsigned long long)wc.nleaf);
+    atomicAdd(A.evals + 3, (unsigned long long)wc.nstep);
   }
-  if (singular) atomicOr(A.flags, kFlagSingular);
+  if (wc.singular) atomicOr(A.flags, kFlagSingular);
   if (!valid) return;
   double us[3] = {0., 0., 0.};
   if (kMode == QV_FP64) {
-    if (S.n == 1) {
-      uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2];
-    } else {
-      us[0] = u[0]; us[1] = u[1]; us[2] = u[2];
-    }
+    us[0] = u[0]; us[1] = u[1]; us[2] = u[2];
   } else {
-    if (S.n == 1) acc.get(uc[0], uc[1], uc[2]);
-    else acc.get(us[0], us[1], us[2]);
+    acc.get(us[0], us[1], us[2]);
   }
   // u_bs * quant_circ / 4 / pi (tree.f90:390, 411), once per sum
   for (int k = 0; k < 3; ++k) {
@@ -354,9 +365,9 @@ hipError_t launch_walk(Ctx &c) {
   int g = (c.nact + kBlock - 1) / kBlock;
   g = ((g + 7) / 8) * 8;
   if (c.p.precision == QV_FP32C)
-    hipLaunchKernelGGL((k_walk<QV_FP32C, AccF64>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
+    hipLaunchKernelGGL((k_walk<QV_FP32C>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
   else
-    hipLaunchKernelGGL((k_walk<QV_FP64, AccF64>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
+    hipLaunchKernelGGL((k_walk<QV_FP64>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
   return hipGetLastError();
 }
 

@@ -31,7 +31,8 @@ QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE,
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
            "qv_velocity", "qv_ab_step", "qv_enforce_periodic", "qv_step", "qv_download",
-           "qv_tree_dump", "qv_get_timings", "qv_status_string"]
+           "qv_tree_dump", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
+           "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_status_string"]
 
 
 class QvParams(C.Structure):
@@ -92,6 +93,12 @@ def lib():
         L.qv_download.argtypes = [vp, d, d, d, d, d]
         L.qv_tree_dump.argtypes = [vp, d, C.c_int64, i64]
         L.qv_get_timings.argtypes = [vp, P(QvTimings)]
+        L.qv_vjp.argtypes = [vp, d, d]
+        L.qv_jvp.argtypes = [vp, d, d]
+        L.qv_set_stream.argtypes = [vp, vp]
+        L.qv_velocity_device.argtypes = [vp, vp, vp]
+        L.qv_vjp_device.argtypes = [vp, vp, vp, vp]
+        L.qv_jvp_device.argtypes = [vp, vp, vp, vp]
         L.qv_status_string.argtypes = [C.c_int]
         L.qv_status_string.restype = C.c_char_p
         for name in EXPORTS[:-1]:
@@ -203,6 +210,32 @@ class Context:
         out = np.zeros((nn.value, 22))
         _check(lib().qv_tree_dump(self._h, _dp(out), nn.value, C.byref(nn)), "qv_tree_dump")
         return out
+
+    def vjp(self, w) -> np.ndarray:
+        """xbar = (dU/dx)^T w at the device positions (direct-sum velocity)."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = np.zeros_like(w)
+        _check(lib().qv_vjp(self._h, _dp(w), _dp(out)), "qv_vjp")
+        return out
+
+    def jvp(self, v) -> np.ndarray:
+        """jv = (dU/dx) v at the device positions (direct-sum velocity)."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(v)
+        _check(lib().qv_jvp(self._h, _dp(v), _dp(out)), "qv_jvp")
+        return out
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        _check(lib().qv_set_stream(self._h, stream_handle), "qv_set_stream")
+
+    def velocity_device(self, d_x: int | None, d_u: int) -> None:
+        _check(lib().qv_velocity_device(self._h, d_x, d_u), "qv_velocity_device")
+
+    def vjp_device(self, d_x: int | None, d_w: int, d_out: int) -> None:
+        _check(lib().qv_vjp_device(self._h, d_x, d_w, d_out), "qv_vjp_device")
+
+    def jvp_device(self, d_x: int | None, d_v: int, d_out: int) -> None:
+        _check(lib().qv_jvp_device(self._h, d_x, d_v, d_out), "qv_jvp_device")
 
     def timings(self) -> dict:
         t = QvTimings()
