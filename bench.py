@@ -14,6 +14,12 @@ HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--points N]
 
+Also measured, under "ensemble": 256 independent vortex rings in open
+boundaries (point counts U{200..5000}, radius U[0.02, 0.06] cm, helical
+perturbation of 1% of R), FP32 pair terms with FP64 LIA, advanced 500 AB
+steps in persistent launches replayed from a captured HIP graph, with the
+tile shape picked by the library's autotuner.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -119,6 +125,53 @@ def roofline(res, precision: str, n: int, traffic):
                          "peak": HBM_PEAK_GBS}}
 
 
+PAIR_FLOPS = 52  # one FP32 segment term, counting FMA as 2: see run_ensemble
+
+
+def run_ensemble(steps: int, seed: int = 20261202):
+    """Ensemble of 256 rings advanced together (qv_batch_*).  dt is half the
+    stability bound of timestep_check (initial.f90:606-622) for the finest
+    spacing in the ensemble.  One pair term of the FP32 path costs 52 flops:
+    relative vector 3, R1 x e 9, |R1 x e|^2 5, R2 = R1 + e 3, |R1|^2 and
+    |R2|^2 10, two rsqrt 2, |R1|, |R2|, |R1||R2| 3, R1.R2 5, g 3, f 3, FP64
+    accumulation 6."""
+    import math
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    systems = G.ring_ensemble(256, seed=seed)
+    counts = np.array([s.pcount for s in systems])
+    spacing = min(float(np.min(np.linalg.norm(s.x[s.infront - 1] - s.x, axis=1)))
+                  for s in systems)
+    dmin = spacing / 2.0
+    dt = 0.5 * abs(dmin ** 2 / (9.97e-4 * math.log(dmin / (math.pi * 1.3e-8))))
+    kw = dict(velocity="BS", boundary="open", box_size=0.2, corea=1.3e-8, quant_circ=9.97e-4,
+              dt=dt)
+    with gpu.Batch(gpu.params(precision="fp64", **kw), systems) as b:
+        u64 = np.concatenate(b.velocity())
+    with gpu.Batch(gpu.params(precision="fp32c", **kw), systems) as b:
+        chosen, tune_ms = b.tune(3)
+        u32 = np.concatenate(b.velocity())
+        b.step(10, graph=True)  # warm-up: capture and first replays
+        ms = b.step(steps, graph=True)
+        info = b.info()
+    nr = np.linalg.norm(u64, axis=1)
+    e = np.linalg.norm(u32 - u64, axis=1) / nr
+    n = int(counts.sum())
+    pairs = int(np.sum(counts * (counts - 2)))
+    tflops = PAIR_FLOPS * pairs / (ms / steps * 1e-3) / 1e12
+    return {"workload": "256 rings, open, points U{200..5000}, R U[0.02,0.06] cm, "
+                        "helix 1% of R, fp32 pairs + fp64 LIA, AB steps",
+            "systems": 256, "points": n, "steps": steps, "dt": dt,
+            "ms_per_step": round(ms / steps, 4),
+            "point_velocities_per_s": n * steps / (ms * 1e-3),
+            "pairs_per_step": pairs, "pair_flops": PAIR_FLOPS,
+            "achieved_tflops": round(tflops, 2), "peak_tflops_fp32": FLOP_PEAK_TFS["fp32c"],
+            "frac": round(tflops / FLOP_PEAK_TFS["fp32c"], 4),
+            "graph": True, "variant": chosen, "targets_per_unit": info["tile"],
+            "work_units": info["nunits"], "tune_ms": [round(t, 4) for t in tune_ms],
+            "error_vs_fp64": {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e ** 2)))}}
+
+
 def copy_calibration(gib: float = 2.0, reps: int = 5) -> None:
     """A plain device copy of known size (torch), for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -142,6 +195,8 @@ def main():
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32c"], default=None,
                     help="profile one precision (no JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -216,6 +271,10 @@ def main():
         "roofline": roofline(r32, "fp32c", n, traffic),
         "roofline_fp64": roofline(r64, "fp64", n, None),
     }
+    if not args.no_ensemble:
+        out["ensemble"] = run_ensemble(args.ensemble_steps)
+        print(f"ensemble: {out['ensemble']['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fil, args.theta, args.cpu_sample)
     print(json.dumps(out), flush=True)

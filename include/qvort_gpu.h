@@ -41,6 +41,7 @@ extern "C" {
 #endif
 
 typedef struct qv_ctx qv_ctx;
+typedef struct qv_batch qv_batch;
 
 /* run.in `velocity` (cdata.f90:489-491) */
 enum qv_velocity { QV_VEL_BS = 1, QV_VEL_TREE = 2 };
@@ -126,6 +127,24 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2,
  * are written (out may be NULL to query the count). */
 int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes);
 
+/* pclose_tree / closest_tree (tree.f90:219-278) when tree_theta > 0, else
+ * pclose (line.f90:205-234): for every slot, the closest point that is not
+ * i, infront(i) or behind(i), with the context's current positions and
+ * links.  The tree is the one of the last qv_velocity (construct_tree runs
+ * whenever tree_theta > 0, run.f90:43-45, also with velocity='BS'), built
+ * from the positions before that step's motion: the reference's order is
+ * construct_tree, pmotion, pinsert, pclose_tree (run.f90:43-66), so a caller
+ * may qv_upload the state after its host pinsert (more slots, new links)
+ * before this call.  A one-point leaf offers j = behind(infront(p)) with
+ * infront as at the build and behind as now; a many-point cell taken by
+ * width/dist < 0.5 sets closest = 0.  closest (1-based) and closestd are
+ * f(:)%closest / %closestd, in and out: tree mode sets closestd = 100 for
+ * every slot before searching; a slot where nothing is taken keeps its
+ * closest; pclose leaves empty slots untouched.  Bit-identical to the
+ * reference.  QV_ERR_STATE: tree mode without a tree, or fewer slots than
+ * at the build. */
+int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd);
+
 /* ---- derivatives of the direct-sum velocity (velocity = QV_VEL_BS) ----
  * No counterpart in the reference: the adjoint, for gradients of objectives
  * (line length, energy) through many steps, and the tangent map, for
@@ -149,6 +168,38 @@ int qv_set_stream(qv_ctx *ctx, void *hip_stream);
 int qv_velocity_device(qv_ctx *ctx, const double *d_x, double *d_u);
 int qv_vjp_device(qv_ctx *ctx, const double *d_x, const double *d_w, double *d_xbar);
 int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_jv);
+
+/* ---- ensembles of independent systems.  No counterpart in the reference,
+ * which runs one system per process (run.f90).  Many filament systems of
+ * uneven point counts, velocity='BS' with the same run.in parameters, are
+ * held in one context and advanced together: one persistent launch per
+ * step computes every system's LIA term + direct sum and applies each
+ * system's AB ladder and wrap.  precision QV_FP64 gives, per system, the
+ * velocity qv_velocity gives for that system alone (bit-identical);
+ * QV_FP32C uses FP32 pair terms with FP64 relative vectors and sums.
+ * Arrays are the systems' x(3,n_s) arrays one after another; links are
+ * 1-based within each system. */
+int qv_batch_create(const qv_params *params, int32_t nsys, const int32_t *counts,
+                    qv_batch **out);
+int qv_batch_destroy(qv_batch *batch);
+/* positions and links; clears the AB history */
+int qv_batch_upload(qv_batch *batch, const double *x, const int32_t *infront,
+                    const int32_t *behind);
+/* velocity of every system at the current positions (u may be NULL) */
+int qv_batch_velocity(qv_batch *batch, double *u);
+/* nsteps of velocity + AB ladder + wrap; use_graph != 0 replays a captured
+ * HIP graph of up to 10 steps */
+int qv_batch_step(qv_batch *batch, int32_t nsteps, int32_t use_graph);
+/* positions and the last velocity (either may be NULL) */
+int qv_batch_download(qv_batch *batch, double *x, double *u);
+/* times each tile shape on trial_steps velocity launches and keeps the
+ * fastest; ms (may be NULL) receives one time per shape */
+int qv_batch_tune(qv_batch *batch, int32_t trial_steps, int32_t *chosen, float *ms);
+int qv_batch_set_variant(qv_batch *batch, int32_t variant);
+/* number of tile shapes, current shape and its targets per work unit,
+ * work units per launch, device time of the last qv_batch_step call */
+int qv_batch_info(qv_batch *batch, int32_t *nvariants, int32_t *variant, int32_t *tile,
+                  int32_t *nunits, float *last_ms);
 
 int qv_get_timings(qv_ctx *ctx, qv_timings *t);
 const char *qv_status_string(int status);

@@ -65,6 +65,11 @@ def lib():
         L.qvo_segment_velocity.restype = C.c_int
         L.qvo_enforce_periodic.argtypes = [P(QvoParams), C.c_int32, d, i32]
         L.qvo_enforce_periodic.restype = C.c_int
+        L.qvo_closest_tree.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, C.c_int32, d, i32,
+                                       i32, i32, d, C.c_int32]
+        L.qvo_closest_tree.restype = C.c_int
+        L.qvo_closest_direct.argtypes = [C.c_int32, d, i32, i32, i32, d, C.c_int32]
+        L.qvo_closest_direct.restype = C.c_int
         _lib = L
     return _lib
 
@@ -160,3 +165,23 @@ def segment_velocity(params: QvoParams, xi, xj, gj, shift=(0.0, 0.0, 0.0), form:
     u = np.zeros(3)
     lib().qvo_segment_velocity(C.byref(params), *[_d(v) for v in a], form, _d(u))
     return u
+
+
+def closest(params: QvoParams, build, current, closest_in, closestd_in, nthreads: int = 0):
+    """Neighbour search of the reconnection step: pclose_tree over the tree
+    of ``build`` (tree_theta > 0) or pclose (tree_theta = 0) with the
+    positions and links of ``current``; returns (closest, closestd)."""
+    x, inf, beh = _prep(current)
+    n = x.shape[0]
+    cl = np.array(closest_in, dtype=np.int32, order="C").copy()
+    cd = np.array(closestd_in, dtype=np.float64, order="C").copy()
+    assert cl.shape == (n,) and cd.shape == (n,)
+    if params.tree_theta > 0:
+        xb, ib, bb = _prep(build)
+        rc = lib().qvo_closest_tree(C.byref(params), xb.shape[0], _d(xb), _i(ib), _i(bb), n,
+                                    _d(x), _i(inf), _i(beh), _i(cl), _d(cd), nthreads)
+    else:
+        rc = lib().qvo_closest_direct(n, _d(x), _i(inf), _i(beh), _i(cl), _d(cd), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"closest search failed ({rc})")
+    return cl, cd

@@ -752,3 +752,94 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
   free(u);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* neighbour search: pclose_tree / closest_tree (tree.f90:219-278) and */
+/* pclose (line.f90:205-234)                                           */
+
+typedef struct {
+  const qvo_tree *t;
+  const int32_t *inf_b;            /* links of the build state */
+  const double *x;                 /* current state */
+  const int32_t *inf, *beh;
+} qvo_close;
+
+static void closest_tree(const qvo_close *c, int64_t ni, int i, int32_t *closest,
+                         double *closestd) {
+  const qvo_node *nd = &c->t->n[ni];
+  if (c->inf[i] == 0) return;                    /* :247 */
+  if (nd->pcount == 0) return;                   /* :248 */
+  const double *xi = c->x + 3 * i;
+  const double dist = sqrt((sq(xi[0] - nd->cent[0]) + sq(xi[1] - nd->cent[1])) +
+                           sq(xi[2] - nd->cent[2]));   /* :250 */
+  const double theta = nd->width / dist;
+  if (nd->pcount == 1 || theta < .5) {
+    if (dist < *closestd) {
+      if (nd->pcount > 1) {
+        *closest = 0;                            /* :258, closestd kept */
+      } else {
+        /* j = f(parray(1)%infront)%behind: build-time infront, current behind */
+        const int32_t q = c->inf_b[nd->rec];
+        const int32_t j = q > 0 ? c->beh[q - 1] : 0;  /* q = 0 only for a 1-slot root */
+        if ((i + 1 != j) && (c->inf[i] != j) && (c->beh[i] != j)) {
+          *closest = j;
+          *closestd = dist;
+        }
+      }
+    }
+  } else {
+    for (int k = 0; k < 8; ++k) closest_tree(c, nd->child[k], i, closest, closestd);
+  }
+}
+
+int qvo_closest_tree(const qvo_params *p, int32_t pcount_b, const double *x_b,
+                     const int32_t *infront_b, const int32_t *behind_b,
+                     int32_t pcount, const double *x, const int32_t *infront,
+                     const int32_t *behind, int32_t *closest, double *closestd,
+                     int32_t nthreads) {
+  if (!p || pcount_b < 1 || pcount < pcount_b || !(p->box_size > 0.)) return -1;
+  qvo_state s;
+  if (state_init(&s, p, pcount_b, x_b, infront_b, behind_b)) { state_free(&s); return -1; }
+  qvo_tree t = {0};
+  if (construct_tree(&s, &t)) { state_free(&s); free(t.n); return -1; }
+  qvo_close c = {&t, infront_b, x, infront, behind};
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < pcount; ++i) {
+    closestd[i] = 100.;                          /* :227, before the empty test */
+    if (infront[i] == 0) continue;
+    closest_tree(&c, 0, i, &closest[i], &closestd[i]);
+  }
+  free(t.n);
+  state_free(&s);
+  return 0;
+}
+
+int qvo_closest_direct(int32_t pcount, const double *x, const int32_t *infront,
+                       const int32_t *behind, int32_t *closest, double *closestd,
+                       int32_t nthreads) {
+  if (pcount < 1) return -1;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < pcount; ++i) {
+    if (infront[i] == 0) continue;               /* :211, closestd kept */
+    double cd = 100.;
+    int32_t cl = closest[i];
+    const double *xi = x + 3 * i;
+    for (int j = 0; j < pcount; ++j) {
+      if (infront[j] == 0) continue;
+      if ((i != j) && (infront[i] != j + 1) && (behind[i] != j + 1)) {
+        const double *xj = x + 3 * j;
+        /* distf (general.f90:27-34) */
+        const double dist = sqrt((sq(xi[0] - xj[0]) + sq(xi[1] - xj[1])) + sq(xi[2] - xj[2]));
+        if (dist < cd) {
+          cl = j + 1;
+          cd = dist;
+        }
+      }
+    }
+    closest[i] = cl;
+    closestd[i] = cd;
+  }
+  return 0;
+}

@@ -87,6 +87,27 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
             double *u1, double *u2, double *u3, int32_t nsteps,
             double *u_first, int32_t nthreads);
 
+/* neighbour search of the reconnection step.
+ * qvo_closest_tree: pclose_tree / closest_tree (tree.f90:219-278).  The tree
+ * is built from the build state (x_b, infront_b, behind_b: the positions and
+ * links of construct_tree, before pmotion and pinsert); the search uses the
+ * current state (pcount >= pcount_b slots).  For every slot closestd = 100,
+ * then an active slot i walks the tree: a node with one point, or with
+ * width/dist < 0.5, is taken when dist < closestd; a many-point node sets
+ * closest = 0 (closestd unchanged); a one-point node's candidate is
+ * j = behind(infront_b(point)) with the current behind, taken unless j is i,
+ * infront(i) or behind(i).  closest is in/out (kept where nothing is taken).
+ * qvo_closest_direct: pclose (line.f90:205-234), all pairs of the current
+ * state; empty slots keep closest and closestd.  Returns 0 or -1. */
+int qvo_closest_tree(const qvo_params *p, int32_t pcount_b, const double *x_b,
+                     const int32_t *infront_b, const int32_t *behind_b,
+                     int32_t pcount, const double *x, const int32_t *infront,
+                     const int32_t *behind, int32_t *closest, double *closestd,
+                     int32_t nthreads);
+int qvo_closest_direct(int32_t pcount, const double *x, const int32_t *infront,
+                       const int32_t *behind, int32_t *closest, double *closestd,
+                       int32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,8 +10,13 @@
 !>   pmotion         timestep.f90:19-93 (calc_velocity timestep.f90:180-370)
 !>   enforce_periodic* periodic.f90:188-303
 !>   empty_tree      tree.f90:189-203
+!>   pinsert         line.f90:18-108      (flags bit 3)
+!>   pclose_tree     tree.f90:219-278     (flags bit 2, tree_theta > 0)
+!>   pclose          line.f90:205-234     (flags bit 2, tree_theta = 0)
 !> and mirrors the order of the main loop in run.f90:36-169 with the line
-!> operations (pinsert/precon/premove) left out.
+!> operations left out, except in the neighbour-search mode (flags bit 2):
+!> one step of ghostp, construct_tree, pmotion, [pinsert], then the search,
+!> as run.f90:41-66 orders them (the tree is the pre-motion one).
 !>
 !> Files (stream access, little endian), in the working directory:
 !>   run.in     read by the reference's read_run_file (cdata.f90:420-807)
@@ -23,12 +28,17 @@
 !>              real64 x, u1, u2, u3                        (3,pcount each, after nsteps)
 !>   tree.bin   (flags bit 0) every node of the first step's tree in DFS order
 !>   timing.txt (flags bit 1) build/walk wall times of a target sample
+!>   closest.bin (flags bit 2) int32 pcount after pinsert;
+!>              real64 x(3,pcount); int32 infront, behind, closest (before the
+!>              search); real64 closestd (before); int32 closest (after);
+!>              real64 closestd (after)
 program qv_ref_driver
   use cdata
   use general
   use periodic
   use tree
   use timestep
+  use line
   use omp_lib
   implicit none
   integer :: version, nsteps_run, flags, nsample
@@ -73,6 +83,10 @@ program qv_ref_driver
 
   if (btest(flags, 1)) then
     call timing_run()
+    stop
+  end if
+  if (btest(flags, 2)) then
+    call closest_run()
     stop
   end if
 
@@ -170,6 +184,33 @@ contains
       call dump_node(nd%bbl, level+1) ; call dump_node(nd%bbr, level+1)
       call dump_node(nd%btl, level+1) ; call dump_node(nd%btr, level+1)
     end if
+  end subroutine
+
+  !> one step up to the neighbour search, in the order of run.f90:41-66
+  subroutine closest_run()
+    itime = 1
+    t = 0.
+    call ghostp
+    if (tree_theta > 0) call construct_tree
+    call pmotion
+    if (btest(flags, 3)) call pinsert
+    open(unit=45, file='closest.bin', access='stream', form='unformatted', status='replace')
+    write(45) pcount
+    do k = 1, pcount
+      write(45) f(k)%x
+    end do
+    write(45) (f(k)%infront, k = 1, pcount)
+    write(45) (f(k)%behind, k = 1, pcount)
+    write(45) (f(k)%closest, k = 1, pcount)
+    write(45) (f(k)%closestd, k = 1, pcount)
+    if (tree_theta > 0) then
+      call pclose_tree
+    else
+      call pclose
+    end if
+    write(45) (f(k)%closest, k = 1, pcount)
+    write(45) (f(k)%closestd, k = 1, pcount)
+    close(45)
   end subroutine
 
   !> time construct_tree and calc_velocity over a strided sample of targets

@@ -98,3 +98,44 @@ def parse_tree(raw: bytes) -> np.ndarray:
     out[:, 10:13] = rec["circ"]
     out[:, 13:22] = rec["C"]
     return out
+
+
+def closest_run(fil, *, pinsert=False, threads=1, timeout=600, **params) -> dict:
+    """One step of ghostp, construct_tree, pmotion, [pinsert], then pclose_tree
+    (tree_theta > 0) or pclose, in the order of run.f90:41-66.  Returns the
+    state the search saw (pcount, x, infront, behind after pinsert), the
+    closest/closestd arrays before the search and after it."""
+    if not available():
+        raise FileNotFoundError(DRIVER)
+    flags = 4 | (8 if pinsert else 0)
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory(prefix="qvref_") as wd:
+        with open(os.path.join(wd, "run.in"), "w") as fh:
+            fh.write(run_in_text(nsteps=1, **params))
+        os.makedirs(os.path.join(wd, "data"), exist_ok=True)
+        _write_state(os.path.join(wd, "state.bin"), fil, 1, flags, 0)
+        proc = subprocess.run([DRIVER], cwd=wd, env=env, capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0 or "FATAL" in proc.stdout:
+            raise RuntimeError(f"reference driver failed: {proc.stdout[-2000:]}{proc.stderr[-2000:]}")
+        raw = open(os.path.join(wd, "closest.bin"), "rb").read()
+    n = int(np.frombuffer(raw, dtype="<i4", count=1)[0])
+    off = 4
+    out = {"pcount": n}
+
+    def take(dtype, count, shape=None):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        return a if shape is None else a.reshape(shape)
+
+    out["x"] = take("<f8", 3 * n, (n, 3))
+    out["infront"] = take("<i4", n)
+    out["behind"] = take("<i4", n)
+    out["closest_in"] = take("<i4", n)
+    out["closestd_in"] = take("<f8", n)
+    out["closest"] = take("<i4", n)
+    out["closestd"] = take("<f8", n)
+    assert off == len(raw)
+    return out

@@ -4,6 +4,6 @@ Native library: qvort_amd/lib/libqvort_gpu.so (C ABI in include/qvort_gpu.h),
 bound here by :mod:`qvort_amd.gpu`; Fortran bindings in qvort_amd/fortran.
 """
 from . import geometry  # noqa: F401
-from .gpu import Context, QvortError, params  # noqa: F401
+from .gpu import Batch, Context, QvortError, params  # noqa: F401
 
-__all__ = ["Context", "QvortError", "params", "geometry"]
+__all__ = ["Batch", "Context", "QvortError", "params", "geometry"]

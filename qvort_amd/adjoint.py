@@ -174,7 +174,7 @@ def gradient(objective: str, x0, eng: Engine, nsteps: int, dt: float,
     xf = rollout(x, eng, nsteps, dt, checkpoint_every)
     J = line_length(eng, xf) if objective == "length" else energy(eng, xf)
     (g,) = torch.autograd.grad(J, x)
-    return float(J), g.detach().cpu().numpy()
+    return float(J.detach()), g.detach().cpu().numpy()
 
 
 def objective_value(objective: str, x0, eng: Engine, nsteps: int, dt: float) -> float:

@@ -101,15 +101,19 @@ int velocity_stage(Ctx &c) {
   TRY(launch_ghost_lia(c));
   TRY(hipEventRecord(c.ev[1], c.stream));
   int rc = QV_OK;
+  // construct_tree runs whenever tree_theta > 0 (run.f90:43-45), also with
+  // velocity='BS', where only the neighbour search uses it
+  c.tree_valid = false;
+  if (c.p.tree_theta > 0. && c.nact > 0) {
+    rc = tree_build(c);
+    if (rc != QV_OK) return rc;
+  } else {
+    c.tree.nnodes = 0;
+  }
+  TRY(hipEventRecord(c.ev[2], c.stream));
   if (c.p.velocity == QV_VEL_TREE) {
-    if (c.nact > 0) {
-      rc = tree_build(c);
-      if (rc != QV_OK) return rc;
-    }
-    TRY(hipEventRecord(c.ev[2], c.stream));
     if (c.nact > 0) TRY(launch_walk(c));
   } else {
-    TRY(hipEventRecord(c.ev[2], c.stream));
     TRY(launch_direct(c));
   }
   TRY(hipEventRecord(c.ev[3], c.stream));
@@ -126,7 +130,7 @@ int velocity_stage(Ctx &c) {
   c.timings.walk_ms = ms;
   (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[3]);
   c.timings.total_ms = ms;
-  c.timings.nodes = c.p.velocity == QV_VEL_TREE ? c.tree.nnodes : 0;
+  c.timings.nodes = c.p.tree_theta > 0. ? c.tree.nnodes : 0;
   if (c.p.velocity == QV_VEL_TREE) {
     // eval_counter starts at 2N (tree.f90:77-78) and counts every leaf and
     // accepted-cell interaction
@@ -145,6 +149,8 @@ int velocity_stage(Ctx &c) {
   }
   if (c.h_counters[1] & kFlagSingular) return QV_ERR_SINGULAR;
   c.have_velocity = true;
+  c.tree_valid = c.p.tree_theta > 0.;
+  c.tree_pcount = c.pcount;
   return QV_OK;
 }
 
@@ -303,7 +309,8 @@ int qv_destroy(qv_ctx *ctx) {
                 c->slot_s, c->nnode_of, c->node_off, c->sort_tmp, c->counters, c->evals,
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
-                c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner};
+                c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
+                c->tree.linf, c->close_i, c->close_d};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->h_evals) (void)hipHostFree(c->h_evals);
@@ -438,10 +445,34 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, doubl
   return QV_OK;
 }
 
+int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
+  if (!ctx || !closest || !closestd) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  const bool tree = c->p.tree_theta > 0.;
+  // the tree must exist and its leaves' slots must still be slots
+  if (tree && (!c->tree_valid || c->pcount < c->tree_pcount)) return QV_ERR_STATE;
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  if (c->close_cap < c->pcount) {
+    TRY(hipStreamSynchronize(c->stream));
+    TRY(dalloc(c->close_i, c->pcount));
+    TRY(dalloc(c->close_d, c->pcount));
+    c->close_cap = c->pcount;
+  }
+  const size_t n = c->pcount;
+  TRY(hipMemcpyAsync(c->close_i, closest, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  TRY(hipMemcpyAsync(c->close_d, closestd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  TRY(launch_closest(*c, tree));
+  TRY(hipMemcpyAsync(closest, c->close_i, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  TRY(hipMemcpyAsync(closestd, c->close_d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
 int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes) {
   if (!ctx) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  if (c->p.velocity != QV_VEL_TREE || c->tree.nnodes == 0) return QV_ERR_STATE;
+  if (!c->tree_valid || c->tree.nnodes == 0) return QV_ERR_STATE;
   const int64_t nn = c->tree.nnodes;
   std::vector<int8_t> level(nn), digit(nn);
   std::vector<int32_t> count(nn);

@@ -65,6 +65,10 @@ struct Tree {
   int8_t *level = nullptr;
   int8_t *digit = nullptr;   // child index within the parent (0..7)
   double *corner = nullptr;  // 3 per node
+  // neighbour search: leaf -> 0-based slot infront(p) - 1 of its point p at
+  // build time (the search resolves j = behind(that slot) with the links of
+  // its own call); other nodes -1
+  int32_t *linf = nullptr;
 };
 
 struct Ctx {
@@ -75,6 +79,14 @@ struct Ctx {
   int32_t pcount = 0, cap = 0;
   int32_t nact = 0;
   bool have_velocity = false;
+  // the tree of the last velocity stage (built whenever tree_theta > 0, as
+  // construct_tree is, run.f90:43-45) stays usable by qv_closest after later
+  // uploads; tree_pcount = slots at build time
+  bool tree_valid = false;
+  int32_t tree_pcount = 0;
+  int32_t *close_i = nullptr;   // closest (1-based), per slot
+  double *close_d = nullptr;    // closestd, per slot
+  int32_t close_cap = 0;
   // state (SoA), slot order
   Vec3 x, gi, gb, gii, gbb, u, uli, ubs, u1, u2, u3;
   int32_t *infront = nullptr, *behind = nullptr;
@@ -151,5 +163,6 @@ int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
 hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
+hipError_t launch_closest(Ctx &c, bool tree);
 
 }  // namespace qv

@@ -196,6 +196,7 @@ struct NodeArgs {
   double *corner;
   int2 *meta;
   double2 *geo;
+  int32_t *linf;
 };
 
 // end of the level-`lev` cell whose first member is s: first t > s with
@@ -224,6 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_make_nodes(NodeArgs A) {
     int j = -1;
     if (A.pcount == 1 && A.inf[0] != 0) j = A.beh[A.inf[0] - 1] - 1;
     A.meta[0] = make_int2((int)A.nnodes, j);
+    A.linf[0] = (A.pcount == 1 && A.inf[0] != 0) ? A.inf[0] - 1 : -1;
   }
   if (s >= A.nact) return;
   const int c = A.nn[s];
@@ -263,6 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_make_nodes(NodeArgs A) {
     if (e - s == 1) j = A.beh[A.inf[p] - 1] - 1;  // f(parray(1)%infront)%behind
     const int skip = e < A.nact ? A.off[e] : (int)A.nnodes;
     A.meta[node] = make_int2(skip, j);
+    A.linf[node] = e - s == 1 ? A.inf[p] - 1 : -1;
   }
 }
 
@@ -500,6 +503,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   int64_t cap = need + need / 4 + 1024;
   (void)hipFree(t.rec); (void)hipFree(t.meta); (void)hipFree(t.mp); (void)hipFree(t.mp32); (void)hipFree(t.geo);
   (void)hipFree(t.first); (void)hipFree(t.count); (void)hipFree(t.level); (void)hipFree(t.digit); (void)hipFree(t.corner);
+  (void)hipFree(t.linf);
   t = Tree{};
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&t.rec, cap * sizeof(NodeRec)))) return e;
@@ -512,6 +516,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   if ((e = hipMalloc(&t.level, cap * sizeof(int8_t)))) return e;
   if ((e = hipMalloc(&t.digit, cap * sizeof(int8_t)))) return e;
   if ((e = hipMalloc(&t.corner, cap * 3 * sizeof(double)))) return e;
+  if ((e = hipMalloc(&t.linf, cap * sizeof(int32_t)))) return e;
   t.cap = cap;
   return hipSuccess;
 }
@@ -604,6 +609,7 @@ int tree_build(Ctx &c) {
   NA.corner = c.tree.corner;
   NA.meta = c.tree.meta;
   NA.geo = c.tree.geo;
+  NA.linf = c.tree.linf;
   hipLaunchKernelGGL(k_make_nodes, dim3(grid_of(nact > 0 ? nact : 1)), dim3(kBlock), 0, st, NA);
   QV_TRY(hipGetLastError());
   // 7. moments, geo distance, acceptance thresholds

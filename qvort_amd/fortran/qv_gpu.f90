@@ -7,7 +7,10 @@
 !>   velocity loop of pmotion (timestep.f90:41-47),
 !> qv_ab_step the Adams-Bashforth ladder of pmotion (timestep.f90:48-71),
 !> qv_enforce_periodic the wrap of enforce_periodic (periodic.f90:188-213),
-!> and qv_destroy the empty_tree / deallocate(vtree) of run.f90:132-138.
+!> qv_closest the neighbour search pclose_tree / pclose (tree.f90:219-278,
+!> line.f90:205-234) that precon reads, and qv_destroy the empty_tree /
+!> deallocate(vtree) of run.f90:132-138.  qv_vjp / qv_jvp (no counterpart in
+!> the reference) give the adjoint and tangent of the direct-sum velocity.
 !> Default reals of a -fdefault-real-8 build are real(c_double); default
 !> integers are integer(c_int32_t).
 module qv_gpu
@@ -37,7 +40,8 @@ module qv_gpu
 
   public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
             qv_velocity, qv_ab_step, qv_enforce_periodic, qv_step, qv_download, &
-            qv_tree_dump, qv_get_timings, qv_status_string, qv_check
+            qv_tree_dump, qv_closest, qv_vjp, qv_jvp, qv_get_timings, qv_status_string, &
+            qv_check
 
   interface
     integer(c_int) function qv_create(params, ctx) bind(C, name='qv_create')
@@ -91,6 +95,24 @@ module qv_gpu
       type(c_ptr), value :: ctx, out
       integer(c_int64_t), value :: max_nodes
       integer(c_int64_t), intent(out) :: nnodes
+    end function
+    integer(c_int) function qv_closest(ctx, closest, closestd) bind(C, name='qv_closest')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(inout) :: closest(*)
+      real(c_double), intent(inout) :: closestd(*)
+    end function
+    integer(c_int) function qv_vjp(ctx, w, xbar) bind(C, name='qv_vjp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: w(3, *)
+      real(c_double), intent(out) :: xbar(3, *)
+    end function
+    integer(c_int) function qv_jvp(ctx, v, jv) bind(C, name='qv_jvp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: v(3, *)
+      real(c_double), intent(out) :: jv(3, *)
     end function
     integer(c_int) function qv_get_timings(ctx, t) bind(C, name='qv_get_timings')
       import :: c_int, c_ptr, qv_timings

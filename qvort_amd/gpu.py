@@ -31,8 +31,11 @@ QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE,
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
            "qv_velocity", "qv_ab_step", "qv_enforce_periodic", "qv_step", "qv_download",
-           "qv_tree_dump", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
-           "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_status_string"]
+           "qv_tree_dump", "qv_closest", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
+           "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_batch_create",
+           "qv_batch_destroy", "qv_batch_upload", "qv_batch_velocity", "qv_batch_step",
+           "qv_batch_download", "qv_batch_tune", "qv_batch_set_variant", "qv_batch_info",
+           "qv_status_string"]
 
 
 class QvParams(C.Structure):
@@ -71,12 +74,26 @@ def params(*, quant_circ=9.97e-4, corea=8.244023e-9, box_size=0.1, delta=5e-4, t
 _lib = None
 
 
+def _one_hip_runtime() -> None:
+    """PyTorch ships its own libamdhip64.so.7; a process that loads that one
+    and ROCm's copy holds two HIP runtimes, and the second to initialise does
+    not see the device.  Loading and initialising torch (when installed)
+    before the library makes the library bind torch's runtime, so the ctypes
+    calls and torch tensors share one runtime, in either order of use."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     """Load the library (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError(f"{LIB_PATH} missing: run `python -m qvort_amd.build`")
+        _one_hip_runtime()
         L = C.CDLL(LIB_PATH)
         P = C.POINTER
         d, i32, i64 = P(C.c_double), P(C.c_int32), P(C.c_int64)
@@ -93,12 +110,22 @@ def lib():
         L.qv_download.argtypes = [vp, d, d, d, d, d]
         L.qv_tree_dump.argtypes = [vp, d, C.c_int64, i64]
         L.qv_get_timings.argtypes = [vp, P(QvTimings)]
+        L.qv_closest.argtypes = [vp, i32, d]
         L.qv_vjp.argtypes = [vp, d, d]
         L.qv_jvp.argtypes = [vp, d, d]
         L.qv_set_stream.argtypes = [vp, vp]
         L.qv_velocity_device.argtypes = [vp, vp, vp]
         L.qv_vjp_device.argtypes = [vp, vp, vp, vp]
         L.qv_jvp_device.argtypes = [vp, vp, vp, vp]
+        L.qv_batch_create.argtypes = [P(QvParams), C.c_int32, i32, P(vp)]
+        L.qv_batch_destroy.argtypes = [vp]
+        L.qv_batch_upload.argtypes = [vp, d, i32, i32]
+        L.qv_batch_velocity.argtypes = [vp, d]
+        L.qv_batch_step.argtypes = [vp, C.c_int32, C.c_int32]
+        L.qv_batch_download.argtypes = [vp, d, d]
+        L.qv_batch_tune.argtypes = [vp, C.c_int32, i32, P(C.c_float)]
+        L.qv_batch_set_variant.argtypes = [vp, C.c_int32]
+        L.qv_batch_info.argtypes = [vp, i32, i32, i32, i32, P(C.c_float)]
         L.qv_status_string.argtypes = [C.c_int]
         L.qv_status_string.restype = C.c_char_p
         for name in EXPORTS[:-1]:
@@ -211,6 +238,20 @@ class Context:
         _check(lib().qv_tree_dump(self._h, _dp(out), nn.value, C.byref(nn)), "qv_tree_dump")
         return out
 
+    def closest(self, closest_in=None, closestd_in=None):
+        """Neighbour search of the reconnection step (pclose_tree with
+        tree_theta > 0, else pclose) on the current slots: (closest, closestd),
+        from the f(:)%closest / %closestd values given (zeros if None)."""
+        n = self.pcount
+        cl = np.zeros(n, dtype=np.int32) if closest_in is None else \
+            np.array(closest_in, dtype=np.int32, order="C").copy()
+        cd = np.zeros(n) if closestd_in is None else \
+            np.array(closestd_in, dtype=np.float64, order="C").copy()
+        if cl.shape != (n,) or cd.shape != (n,):
+            raise ValueError("closest/closestd must have one entry per slot")
+        _check(lib().qv_closest(self._h, _ip(cl), _dp(cd)), "qv_closest")
+        return cl, cd
+
     def vjp(self, w) -> np.ndarray:
         """xbar = (dU/dx)^T w at the device positions (direct-sum velocity)."""
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -241,3 +282,88 @@ class Context:
         t = QvTimings()
         _check(lib().qv_get_timings(self._h, C.byref(t)), "qv_get_timings")
         return {k: getattr(t, k) for k, _ in QvTimings._fields_}
+
+
+class Batch:
+    """An ensemble of independent filament systems (velocity='BS') advanced
+    together, one persistent launch per AB step (qv_batch_* entry points).
+    ``systems`` are filament records (x, infront, behind with 1-based links
+    within each system)."""
+
+    def __init__(self, p: QvParams, systems):
+        self._h = C.c_void_p()
+        self.counts = np.array([s.x.shape[0] for s in systems], dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        rc = lib().qv_batch_create(C.byref(p), len(systems), _ip(self.counts),
+                                   C.byref(self._h))
+        if rc != QV_OK:
+            self.close()
+            raise QvortError(rc, "qv_batch_create")
+        self.upload(systems)
+
+    @property
+    def ntot(self) -> int:
+        return int(self.offsets[-1])
+
+    def close(self) -> None:
+        if self._h:
+            lib().qv_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, systems) -> None:
+        x = np.ascontiguousarray(np.concatenate([s.x for s in systems]), dtype=np.float64)
+        inf = np.ascontiguousarray(np.concatenate([s.infront for s in systems]), dtype=np.int32)
+        beh = np.ascontiguousarray(np.concatenate([s.behind for s in systems]), dtype=np.int32)
+        if x.shape != (self.ntot, 3) or inf.shape != (self.ntot,) or beh.shape != (self.ntot,):
+            raise ValueError("systems do not match the counts of this ensemble")
+        _check(lib().qv_batch_upload(self._h, _dp(x), _ip(inf), _ip(beh)), "qv_batch_upload")
+
+    def split(self, a: np.ndarray) -> list:
+        return [a[self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.counts))]
+
+    def velocity(self) -> list:
+        u = np.zeros((self.ntot, 3))
+        _check(lib().qv_batch_velocity(self._h, _dp(u)), "qv_batch_velocity")
+        return self.split(u)
+
+    def step(self, nsteps: int, graph: bool = True) -> float:
+        """nsteps AB steps of every system; returns the device time in ms."""
+        _check(lib().qv_batch_step(self._h, nsteps, 1 if graph else 0), "qv_batch_step")
+        return self.info()["last_ms"]
+
+    def download(self):
+        """(positions, last velocity), each a list of (n_s,3) arrays."""
+        x, u = np.zeros((self.ntot, 3)), np.zeros((self.ntot, 3))
+        _check(lib().qv_batch_download(self._h, _dp(x), _dp(u)), "qv_batch_download")
+        return self.split(x), self.split(u)
+
+    def tune(self, trial_steps: int = 3):
+        """Time every tile shape and keep the fastest: (chosen, ms per shape)."""
+        nv = self.info()["nvariants"]
+        ms = (C.c_float * nv)()
+        ch = C.c_int32(-1)
+        _check(lib().qv_batch_tune(self._h, trial_steps, C.byref(ch), ms), "qv_batch_tune")
+        return int(ch.value), [float(v) for v in ms]
+
+    def set_variant(self, v: int) -> None:
+        _check(lib().qv_batch_set_variant(self._h, v), "qv_batch_set_variant")
+
+    def info(self) -> dict:
+        nv, v, tile, nu = (C.c_int32() for _ in range(4))
+        ms = C.c_float()
+        _check(lib().qv_batch_info(self._h, C.byref(nv), C.byref(v), C.byref(tile), C.byref(nu),
+                                   C.byref(ms)), "qv_batch_info")
+        return {"nvariants": nv.value, "variant": v.value, "tile": tile.value,
+                "nunits": nu.value, "last_ms": ms.value}

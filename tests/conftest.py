@@ -21,6 +21,11 @@ def golden_names():
         return sorted(json.load(fh))
 
 
+def closest_names():
+    with open(os.path.join(GOLDEN, "closest_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
 def load_golden(name):
     z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     d = {k: z[k] for k in z.files}

@@ -5,7 +5,14 @@ built by ``make -C oracle ref`` from /root/reference/src) on small seeded
 inputs and stores inputs and outputs as compressed numpy archives.  Each
 case records the run.in parameters it used.  Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # velocity / tree / AB cases
+    python tests/golden/make_golden.py --closest    # neighbour-search cases
+
+The neighbour-search cases (closest_*.npz, index in closest_index.json) run
+one step of ghostp, construct_tree, pmotion, optionally pinsert, then
+pclose_tree or pclose, and store the build state (x0, infront0, behind0),
+the state the search saw (x, infront, behind), closest/closestd before the
+search (closest_in, closestd_in) and after it.
 """
 from __future__ import annotations
 
@@ -61,7 +68,45 @@ def cases():
     ]
 
 
+def closest_cases():
+    loops = G.random_loops(0.1, 5e-4, n_loops=6, seed=5)
+    holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=8, seed=9), 40, seed=3)
+    loops_open = G.random_loops(0.1, 5e-4, n_loops=5, seed=11, wrap=False)
+    base = dict(box_size=0.1, delta=3.5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4,
+                boundary="periodic")
+    return [
+        ("closest_tree_pinsert", loops, dict(base, velocity="Tree", tree_theta=0.4), True),
+        ("closest_tree_holes", holes, dict(base, velocity="Tree", tree_theta=0.3), True),
+        ("closest_tree_open", loops_open, dict(base, velocity="Tree", tree_theta=0.6,
+                                               boundary="open", box_size=0.2), False),
+        ("closest_bs_pinsert", holes, dict(base, velocity="BS", tree_theta=0.0), True),
+        ("closest_bs_with_tree", loops, dict(base, velocity="BS", tree_theta=0.4), False),
+    ]
+
+
+def main_closest():
+    index = {}
+    for name, fil, params, pinsert in closest_cases():
+        r = refrun.closest_run(fil, pinsert=pinsert, **params)
+        arrays = dict(x0=fil.x, infront0=fil.infront, behind0=fil.behind, x=r["x"],
+                      infront=r["infront"], behind=r["behind"], closest_in=r["closest_in"],
+                      closestd_in=r["closestd_in"], closest=r["closest"],
+                      closestd=r["closestd"])
+        meta = dict(params=params, pinsert=pinsert, pcount0=fil.pcount, pcount=r["pcount"])
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        act = r["infront"] != 0
+        print(name, fil.pcount, "->", r["pcount"], "closest=0:",
+              int(np.sum(r["closest"][act] == 0)), flush=True)
+    with open(os.path.join(OUT, "closest_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def main():
+    if "--closest" in sys.argv[1:]:
+        main_closest()
+        return
     index = {}
     for name, fil, params, nsteps, dump, hist in cases():
         u1, u2, u3 = hist if hist is not None else (None, None, None)

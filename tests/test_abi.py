@@ -54,3 +54,26 @@ def test_no_cpu_fallback_without_device():
     with pytest.raises(gpu.QvortError) as ei:
         gpu.Context(gpu.params(velocity="Tree", tree_theta=0.4))
     assert ei.value.status == gpu.QV_ERR_NO_DEVICE
+
+
+def test_batch_argument_checks_before_device():
+    """Ensemble contexts take velocity='BS' only and systems of >= 3 slots;
+    arguments are checked before any device call."""
+    import numpy as np
+    import torch
+    from qvort_amd import gpu
+    lib = gpu.lib()
+    h = C.c_void_p()
+    counts = np.array([200, 5000, 3], dtype=np.int32)
+    cp = counts.ctypes.data_as(C.POINTER(C.c_int32))
+    tree = gpu.params(velocity="Tree", tree_theta=0.4)
+    assert lib.qv_batch_create(C.byref(tree), 3, cp, C.byref(h)) == gpu.QV_ERR_ARG
+    bs = gpu.params(velocity="BS", boundary="open")
+    assert lib.qv_batch_create(C.byref(bs), 0, cp, C.byref(h)) == gpu.QV_ERR_ARG
+    short = np.array([200, 2], dtype=np.int32)
+    assert lib.qv_batch_create(C.byref(bs), 2, short.ctypes.data_as(C.POINTER(C.c_int32)),
+                               C.byref(h)) == gpu.QV_ERR_ARG
+    assert not h.value
+    if not torch.cuda.is_available():
+        assert lib.qv_batch_create(C.byref(bs), 3, cp, C.byref(h)) == gpu.QV_ERR_NO_DEVICE
+        assert lib.qv_batch_destroy(None) == gpu.QV_OK

@@ -100,7 +100,14 @@ def test_tree_mode_rejected():
 @pytest.mark.parametrize("objective", ["length", "energy"])
 def test_multistep_gradient_against_differences(objective):
     """Kelvin-wave line (512 points, 16 waves), 40 AB steps checkpointed every
-    10: autograd through the VJP against central differences of the rollout."""
+    10: autograd through the VJP against central differences of the rollout.
+
+    The differences are Richardson-extrapolated over h and 2h, (4 D(h) -
+    D(2h)) / 3, which removes their h^2 J'''/6 truncation term: with the
+    curvature on the scale of the point spacing (2e-4 cm) that term alone is
+    ~1e-12.  The three largest components must agree to 1e-6 relative; six
+    random ones (typically 1e-3 of the largest: sums of contributions that
+    cancel) to 1e-6 of max(|fd|, 1e-2 max|g|)."""
     from qvort_amd import adjoint as AD
     from qvort_amd import geometry as G
     fil = G.kelvin_line(512, 0.1, amp_sigma=2e-4, seed=61)
@@ -110,16 +117,23 @@ def test_multistep_gradient_against_differences(objective):
         nsteps = 40
         J, g = AD.gradient(objective, fil.x, eng, nsteps, dt, checkpoint_every=10)
         print(objective, "J", J, "dt", dt, flush=True)
-        rng = np.random.Generator(np.random.PCG64(4))
         scale = np.max(np.abs(g))
-        for i in rng.choice(fil.pcount, size=4, replace=False):
-            for a in (0, 2):
+        top = np.argsort(np.abs(g).ravel())[::-1][:3]
+        comps = [(int(k) // 3, int(k) % 3, True) for k in top]
+        rng = np.random.Generator(np.random.PCG64(4))
+        comps += [(int(i), a, False) for i in rng.choice(fil.pcount, size=3, replace=False)
+                  for a in (0, 2)]
+        for i, a, strict in comps:
+            d = []
+            for h in (H, 2 * H):
                 xp, xm = fil.x.copy(), fil.x.copy()
-                xp[i, a] += H
-                xm[i, a] -= H
-                fd = (AD.objective_value(objective, xp, eng, nsteps, dt) -
-                      AD.objective_value(objective, xm, eng, nsteps, dt)) / (2 * H)
-                print(i, a, g[i, a], fd, flush=True)
-                assert abs(g[i, a] - fd) <= 1e-6 * max(abs(fd), 1e-3 * scale)
+                xp[i, a] += h
+                xm[i, a] -= h
+                d.append((AD.objective_value(objective, xp, eng, nsteps, dt) -
+                          AD.objective_value(objective, xm, eng, nsteps, dt)) / (2 * h))
+            fd = (4 * d[0] - d[1]) / 3
+            print(i, a, strict, g[i, a], fd, d[0], flush=True)
+            floor = 0.0 if strict else 1e-2 * scale
+            assert abs(g[i, a] - fd) <= 1e-6 * max(abs(fd), floor)
     finally:
         eng.close()
