@@ -49,6 +49,8 @@ enum qv_velocity { QV_VEL_BS = 1, QV_VEL_TREE = 2 };
 enum qv_boundary { QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, QV_BC_OPENX = 2, QV_BC_OPENXY = 3 };
 /* arithmetic of the nonlocal sum */
 enum qv_precision { QV_FP64 = 0, QV_FP32C = 1 };
+/* flags of qv_probe_velocity */
+enum qv_probe_flags { QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2 };
 
 enum qv_status {
   QV_OK = 0,
@@ -144,6 +146,34 @@ int qv_tree_dump(qv_ctx *ctx, double *out, int64_t max_nodes, int64_t *nnodes);
  * reference.  QV_ERR_STATE: tree mode without a tree, or fewer slots than
  * at the build. */
 int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd);
+
+/* pmotion_RK3 (timestep.f90:95-176), the integrator of run.in
+ * runge_kutta T (run.f90:50-52): three stages of ghostp + construct_tree +
+ * the velocity (as qv_velocity) with the low-storage update u1 = u dt,
+ * x += u1/3; u1 = -(2/3) u1 + u dt, x += u1; u1 = -u1 + u dt, x += u1/2.
+ * u1 is the Runge-Kutta storage (f(:)%u1); no wrap (qv_enforce_periodic). */
+int qv_rk3_step(qv_ctx *ctx);
+
+/* superfluid velocity at nprobe points y (layout y(3,nprobe)), summed as
+ * mesh_velocity does (timestep.f90:374-452) and velocity_quasip
+ * (hamiltonian.f90:26-97): central shell, then the image shells of the
+ * boundary.  velocity='BS': biot_savart_general / _shift
+ * (timestep.f90:516-563) over the current positions with the ghosts of the
+ * last ghostp -- the velocity stage's, or fresh ones with QV_PROBE_GHOSTP
+ * (mesh_velocity runs after pmotion with the step's ghosts; the
+ * quasiparticles after the later ghostp, run.f90:75) -- bit-identical to the
+ * reference.  grad (may be NULL; BS only) receives du_k/dy_m at
+ * grad[9*q + 3*k + m], the analytic derivative of the same sum: it replaces
+ * the six extra evaluations and central differences of velocity_quasip
+ * (hamiltonian.f90:64-79).  velocity='Tree': tree_walk_general
+ * (tree.f90:425-488) over the tree of the last qv_velocity.  The reference
+ * adds leaves as u/(4 pi) without Gamma and cells without Gamma/(4 pi)
+ * (tree.f90:460, 479); by default both get Gamma/(4 pi), the velocity of
+ * the BS sum, and QV_PROBE_REFERENCE_SCALE keeps the reference's factors.
+ * QV_ERR_STATE: no ghosts (BS) or no tree (Tree) yet.  No counterpart for
+ * the gradient in the reference. */
+int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, double *grad,
+                      int32_t flags);
 
 /* ---- derivatives of the direct-sum velocity (velocity = QV_VEL_BS) ----
  * No counterpart in the reference: the adjoint, for gradients of objectives

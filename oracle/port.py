@@ -70,6 +70,12 @@ def lib():
         L.qvo_closest_tree.restype = C.c_int
         L.qvo_closest_direct.argtypes = [C.c_int32, d, i32, i32, i32, d, C.c_int32]
         L.qvo_closest_direct.restype = C.c_int
+        L.qvo_probe_velocity.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, C.c_int32, d, i32,
+                                         d, C.c_int32, d, C.c_int32, d, C.c_int32]
+        L.qvo_probe_velocity.restype = C.c_int
+        L.qvo_run_rk3.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, C.c_int32, d,
+                                  C.c_int32]
+        L.qvo_run_rk3.restype = C.c_int
         _lib = L
     return _lib
 
@@ -185,3 +191,35 @@ def closest(params: QvoParams, build, current, closest_in, closestd_in, nthreads
     if rc != 0:
         raise RuntimeError(f"closest search failed ({rc})")
     return cl, cd
+
+
+def probe_velocity(params: QvoParams, build, current, ghosti, probes, ref_scaling: bool = True,
+                   nthreads: int = 0) -> np.ndarray:
+    """u at probe points as mesh_velocity sums it: biot_savart_general(_shift)
+    over ``current`` with the ghosts ``ghosti`` (BS), or tree_walk_general over
+    the tree of ``build`` (Tree; ref_scaling keeps the reference's factors)."""
+    x, inf, _ = _prep(current)
+    xb, ib, bb = _prep(build)
+    gi = np.ascontiguousarray(ghosti, dtype=np.float64)
+    y = np.ascontiguousarray(probes, dtype=np.float64)
+    u = np.zeros_like(y)
+    rc = lib().qvo_probe_velocity(C.byref(params), xb.shape[0], _d(xb), _i(ib), _i(bb),
+                                  x.shape[0], _d(x), _i(inf), _d(gi), y.shape[0], _d(y),
+                                  1 if ref_scaling else 0, _d(u), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"probe velocity failed ({rc})")
+    return u
+
+
+def run_rk3(params: QvoParams, fil, nsteps: int, u1=None, nthreads: int = 0):
+    """nsteps of pmotion_RK3 + wrap; returns (x, u1, u_last)."""
+    x, inf, beh = _prep(fil)
+    x = x.copy()
+    n = x.shape[0]
+    u1 = np.zeros((n, 3)) if u1 is None else np.array(u1, dtype=np.float64, order="C")
+    ul = np.zeros((n, 3))
+    rc = lib().qvo_run_rk3(C.byref(params), n, _d(x), _i(inf), _i(beh), _d(u1), nsteps,
+                           _d(ul), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"qvo_run_rk3 failed ({rc})")
+    return x, u1, ul

@@ -753,6 +753,41 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
   return 0;
 }
 
+int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
+                const int32_t *infront, const int32_t *behind, double *u1,
+                int32_t nsteps, double *u_last, int32_t nthreads) {
+  double *u = (double *)malloc((size_t)3 * (size_t)pcount * sizeof(double));
+  if (!u) return -1;
+  const double dt = p->dt;
+  for (int step = 0; step < nsteps; ++step) {
+    for (int stage = 0; stage < 3; ++stage) {
+      int rc = qvo_velocity(p, pcount, x, infront, behind, u, NULL, NULL, NULL, nthreads);
+      if (rc) { free(u); return rc; }
+      for (int i = 0; i < pcount; ++i) {
+        if (infront[i] == 0) continue;
+        double *a = u1 + 3 * i, *xi = x + 3 * i;
+        const double *v = u + 3 * i;
+        for (int k = 0; k < 3; ++k) {
+          if (stage == 0) {                              /* :110, :117 */
+            a[k] = v[k] * dt;
+            xi[k] = xi[k] + (a[k] / 3.);
+          } else if (stage == 1) {                       /* :138, :145 */
+            a[k] = -((2. / 3.) * a[k]) + v[k] * dt;
+            xi[k] = xi[k] + a[k];
+          } else {                                       /* :166, :173 */
+            a[k] = -a[k] + v[k] * dt;
+            xi[k] = xi[k] + 0.5 * a[k];
+          }
+        }
+      }
+    }
+    qvo_enforce_periodic(p, pcount, x, infront);
+  }
+  if (u_last) memcpy(u_last, u, (size_t)3 * pcount * sizeof(double));
+  free(u);
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* neighbour search: pclose_tree / closest_tree (tree.f90:219-278) and */
 /* pclose (line.f90:205-234)                                           */
@@ -841,5 +876,120 @@ int qvo_closest_direct(int32_t pcount, const double *x, const int32_t *infront,
     closest[i] = cl;
     closestd[i] = cd;
   }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* velocity at probe points: mesh_velocity (timestep.f90:374-452)      */
+
+/* biot_savart_general(_shift) term of source i at probe y */
+static void bs_general(const double *y, const double *xi, const double *gi, const double *shift,
+                       double q, double *u) {
+  double xs[3], r[3], e[3], ub[3];
+  for (int k = 0; k < 3; ++k) xs[k] = shift ? xi[k] + shift[k] : xi[k];
+  const double a = dist_gen_sq(y, xs);                       /* :527 / :553 */
+  for (int k = 0; k < 3; ++k) r[k] = xs[k] - y[k];
+  for (int k = 0; k < 3; ++k) e[k] = gi[k] - xi[k];
+  const double b = 2. * dot3(r, e);
+  const double c = dist_gen_sq(gi, xi);
+  const double D = (4. * a) * c - b * b;
+  if (D == 0.) return;                                       /* :531 */
+  cross(r, e, ub);
+  const double den = (2. * QVO_PI) * D;
+  for (int k = 0; k < 3; ++k) ub[k] = (ub[k] * q) / den;
+  const double fac = (2. * c + b) / sqrt((a + b) + c) - b / sqrt(a);
+  for (int k = 0; k < 3; ++k) u[k] = u[k] + ub[k] * fac;
+}
+
+static void walk_general(const qvo_params *p, const qvo_tree *t, int64_t ni, const double *y,
+                         const double *shift, int ref_scaling, double *u) {
+  const qvo_node *nd = &t->n[ni];
+  if (nd->pcount == 0) return;                               /* :436 */
+  const double dist = sqrt((sq(y[0] - (nd->cent[0] + shift[0])) +
+                            sq(y[1] - (nd->cent[1] + shift[1]))) +
+                           sq(y[2] - (nd->cent[2] + shift[2])));
+  if (dist < 2.220446049250313e-16) return;                  /* :441 */
+  const double theta = nd->width / dist;
+  const double q = p->quant_circ;
+  if (nd->pcount == 1) {
+    double R1[3], R2[3], ub[3];
+    for (int k = 0; k < 3; ++k) {
+      R1[k] = (nd->cent[k] + shift[k]) - y[k];
+      R2[k] = ((nd->cent[k] + nd->circ[k]) + shift[k]) - y[k];
+    }
+    const double a = vnorm(R1), b = vnorm(R2);
+    const double c = (a * b) * ((a * b) + dot3(R1, R2));
+    cross(R1, R2, ub);
+    const double f = (a + b) / c;
+    for (int k = 0; k < 3; ++k) {
+      ub[k] = f * ub[k];
+      if (ref_scaling) ub[k] = (ub[k] / 4.) / QVO_PI;        /* :460 */
+      else ub[k] = ((ub[k] * q) / 4.) / QVO_PI;
+      u[k] = u[k] + ub[k];
+    }
+  } else if (theta < p->tree_theta) {
+    double v[3], ub[3];
+    for (int k = 0; k < 3; ++k) v[k] = y[k] - (nd->cent[k] + shift[k]);
+    cross(nd->circ, v, ub);
+    const double d3 = pow3(dist), d5 = pow5(dist);
+    for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
+    for (int k = 0; k < 3; ++k) {
+      for (int l = 0; l < 3; ++l) {
+        const double dcl = (v[0] * nd->C[0][l] + v[1] * nd->C[1][l]) + v[2] * nd->C[2][l];
+        for (int m = 0; m < 3; ++m) {
+          double e = 0.;
+          if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
+          ub[m] = ub[m] - (e * nd->C[k][l]) / d3;
+        }
+        for (int m = 0; m < 3; ++m) {
+          double e = 0.;
+          if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
+          ub[m] = ub[m] + (((3. * e) * v[k]) * dcl) / d5;
+        }
+      }
+    }
+    for (int k = 0; k < 3; ++k)                              /* :479 */
+      u[k] = ref_scaling ? u[k] + ub[k] : u[k] + ((ub[k] * q) / 4.) / QVO_PI;
+  } else {
+    for (int c = 0; c < 8; ++c) walk_general(p, t, nd->child[c], y, shift, ref_scaling, u);
+  }
+}
+
+int qvo_probe_velocity(const qvo_params *p, int32_t pcount_b, const double *x_b,
+                       const int32_t *infront_b, const int32_t *behind_b,
+                       int32_t pcount, const double *x, const int32_t *infront,
+                       const double *gi, int32_t nprobe, const double *probe,
+                       int32_t ref_scaling, double *u, int32_t nthreads) {
+  if (!p || nprobe < 0 || !(p->box_size > 0.)) return -1;
+  double sh[27][3];
+  const int ns = shell_shifts(p, sh);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  if (p->velocity == QVO_VEL_BS) {
+    if (pcount < 1) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int q = 0; q < nprobe; ++q) {
+      double v[3] = {0., 0., 0.};
+      for (int s = 0; s < ns; ++s)
+        for (int i = 0; i < pcount; ++i) {
+          if (infront[i] == 0) continue;                     /* :526 */
+          bs_general(probe + 3 * q, x + 3 * i, gi + 3 * i, s ? sh[s] : NULL, p->quant_circ, v);
+        }
+      for (int k = 0; k < 3; ++k) u[3 * q + k] = v[k];
+    }
+    return 0;
+  }
+  if (pcount_b < 1) return -1;
+  qvo_state st;
+  if (state_init(&st, p, pcount_b, x_b, infront_b, behind_b)) { state_free(&st); return -1; }
+  qvo_tree t = {0};
+  if (construct_tree(&st, &t)) { state_free(&st); free(t.n); return -1; }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int q = 0; q < nprobe; ++q) {
+    double v[3] = {0., 0., 0.};
+    for (int s = 0; s < ns; ++s) walk_general(p, &t, 0, probe + 3 * q, sh[s], ref_scaling, v);
+    for (int k = 0; k < 3; ++k) u[3 * q + k] = v[k];
+  }
+  free(t.n);
+  state_free(&st);
   return 0;
 }

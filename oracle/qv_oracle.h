@@ -87,6 +87,16 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
             double *u1, double *u2, double *u3, int32_t nsteps,
             double *u_first, int32_t nthreads);
 
+/* nsteps of the main loop with the Runge-Kutta integrator (runge_kutta T,
+ * run.f90:50-52): pmotion_RK3 (timestep.f90:95-176) -- three stages of
+ * ghostp, construct_tree and the velocity, with the low-storage update
+ * u1 = u dt; x += u1/3; u1 = -(2/3) u1 + u dt; x += u1; u1 = -u1 + u dt;
+ * x += u1/2 -- then the wrap.  u1 is the Runge-Kutta storage, in/out;
+ * u_last receives the last stage's velocity of the last step. */
+int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
+                const int32_t *infront, const int32_t *behind, double *u1,
+                int32_t nsteps, double *u_last, int32_t nthreads);
+
 /* neighbour search of the reconnection step.
  * qvo_closest_tree: pclose_tree / closest_tree (tree.f90:219-278).  The tree
  * is built from the build state (x_b, infront_b, behind_b: the positions and
@@ -107,6 +117,22 @@ int qvo_closest_tree(const qvo_params *p, int32_t pcount_b, const double *x_b,
 int qvo_closest_direct(int32_t pcount, const double *x, const int32_t *infront,
                        const int32_t *behind, int32_t *closest, double *closestd,
                        int32_t nthreads);
+
+/* superfluid velocity at probe points, as mesh_velocity computes it
+ * (timestep.f90:374-452): central shell, then the image shells of the
+ * boundary in the order (p outer .. r inner), one running sum per probe.
+ * velocity BS: biot_savart_general / _shift (timestep.f90:516-563) over the
+ * current positions x with the ghosts gi given (mesh_velocity runs after
+ * pmotion, so they are the ghosts of the step's ghostp); no exclusions.
+ * velocity Tree: tree_walk_general (tree.f90:425-488) over the tree of the
+ * build state (x_b, links_b).  ref_scaling = 1 keeps the reference's factors
+ * there (leaf u_bs/4/pi without Gamma, cell without Gamma/(4 pi)); 0 applies
+ * Gamma/(4 pi) to both as tree_walk does.  Returns 0 or -1. */
+int qvo_probe_velocity(const qvo_params *p, int32_t pcount_b, const double *x_b,
+                       const int32_t *infront_b, const int32_t *behind_b,
+                       int32_t pcount, const double *x, const int32_t *infront,
+                       const double *gi, int32_t nprobe, const double *probe,
+                       int32_t ref_scaling, double *u, int32_t nthreads);
 
 #ifdef __cplusplus
 }

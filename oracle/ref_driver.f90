@@ -8,11 +8,14 @@
 !>   ghostp          periodic.f90:8-17
 !>   construct_tree  tree.f90:44-80
 !>   pmotion         timestep.f90:19-93 (calc_velocity timestep.f90:180-370)
+!>   pmotion_RK3     timestep.f90:95-176  (run.in runge_kutta T, as run.f90:50-54)
 !>   enforce_periodic* periodic.f90:188-303
 !>   empty_tree      tree.f90:189-203
 !>   pinsert         line.f90:18-108      (flags bit 3)
 !>   pclose_tree     tree.f90:219-278     (flags bit 2, tree_theta > 0)
 !>   pclose          line.f90:205-234     (flags bit 2, tree_theta = 0)
+!>   biot_savart_general(_shift) timestep.f90:516-563, tree_walk_general
+!>                   tree.f90:425-488     (flags bit 4, probe points)
 !> and mirrors the order of the main loop in run.f90:36-169 with the line
 !> operations left out, except in the neighbour-search mode (flags bit 2):
 !> one step of ghostp, construct_tree, pmotion, [pinsert], then the search,
@@ -32,6 +35,12 @@
 !>              real64 x(3,pcount); int32 infront, behind, closest (before the
 !>              search); real64 closestd (before); int32 closest (after);
 !>              real64 closestd (after)
+!>   probes.bin (flags bit 4, read) int32 nprobe; real64 y(3,nprobe)
+!>   probe.bin  (flags bit 4) int32 pcount; real64 x(3,pcount), ghosti(3,pcount)
+!>              as the probe sums saw them; real64 u(3,nprobe).  The sums
+!>              follow mesh_velocity (timestep.f90:374-452): ghostp,
+!>              construct_tree, [pmotion if flags bit 5], then per probe the
+!>              central shell and the image shells.
 program qv_ref_driver
   use cdata
   use general
@@ -89,6 +98,10 @@ program qv_ref_driver
     call closest_run()
     stop
   end if
+  if (btest(flags, 4)) then
+    call probe_run()
+    stop
+  end if
 
   allocate(u_first(3,pcount), uli_first(3,pcount), ubs_first(3,pcount))
   allocate(gi_first(3,pcount), gb_first(3,pcount))
@@ -103,7 +116,11 @@ program qv_ref_driver
       call dump_node(vtree, 0)
       close(42)
     end if
-    call pmotion
+    if (runge_kutta) then
+      call pmotion_RK3
+    else
+      call pmotion
+    end if
     if (itime == 1) then
       do i = 1, pcount
         u_first(:,i) = f(i)%u
@@ -211,6 +228,76 @@ contains
     write(45) (f(k)%closest, k = 1, pcount)
     write(45) (f(k)%closestd, k = 1, pcount)
     close(45)
+  end subroutine
+
+  !> velocity at probe points in the way mesh_velocity sums it
+  subroutine probe_run()
+    integer :: np, q, peri, perj, perk
+    real, allocatable :: y(:,:), up(:,:)
+    open(unit=46, file='probes.bin', access='stream', form='unformatted', status='old', &
+         action='read')
+    read(46) np
+    allocate(y(3,np), up(3,np))
+    read(46) y
+    close(46)
+    itime = 1
+    t = 0.
+    call ghostp
+    if (tree_theta > 0) call construct_tree
+    if (btest(flags, 5)) call pmotion
+    do q = 1, np
+      up(:,q) = 0.
+      select case(velocity)
+        case('BS')
+          call biot_savart_general(y(:,q), up(:,q))
+          if (periodic_bc) then
+            do peri=-1,1 ; do perj=-1,1 ; do perk=-1,1
+              if (peri==0.and.perj==0.and.perk==0) cycle
+              call biot_savart_general_shift(y(:,q), up(:,q), &
+                   (/peri*box_size,perj*box_size,perk*box_size/))
+            end do ; end do ; end do
+          else if (periodic_bc_notx) then
+            do perj=-1,1 ; do perk=-1,1
+              if (perj==0.and.perk==0) cycle
+              call biot_savart_general_shift(y(:,q), up(:,q), (/0.,perj*box_size,perk*box_size/))
+            end do ; end do
+          else if (periodic_bc_notxy) then
+            do perk=-1,1
+              if (perk==0) cycle
+              call biot_savart_general_shift(y(:,q), up(:,q), (/0.,0.,perk*box_size/))
+            end do
+          end if
+        case('Tree')
+          call tree_walk_general(y(:,q), vtree, (/0.,0.,0./), up(:,q))
+          if (periodic_bc) then
+            do peri=-1,1 ; do perj=-1,1 ; do perk=-1,1
+              if (peri==0.and.perj==0.and.perk==0) cycle
+              call tree_walk_general(y(:,q), vtree, (/peri*box_size,perj*box_size,perk*box_size/), &
+                   up(:,q))
+            end do ; end do ; end do
+          else if (periodic_bc_notx) then
+            do perj=-1,1 ; do perk=-1,1
+              if (perj==0.and.perk==0) cycle
+              call tree_walk_general(y(:,q), vtree, (/0.,perj*box_size,perk*box_size/), up(:,q))
+            end do ; end do
+          else if (periodic_bc_notxy) then
+            do perk=-1,1
+              if (perk==0) cycle
+              call tree_walk_general(y(:,q), vtree, (/0.,0.,perk*box_size/), up(:,q))
+            end do
+          end if
+      end select
+    end do
+    open(unit=47, file='probe.bin', access='stream', form='unformatted', status='replace')
+    write(47) pcount
+    do k = 1, pcount
+      write(47) f(k)%x
+    end do
+    do k = 1, pcount
+      write(47) f(k)%ghosti
+    end do
+    write(47) up
+    close(47)
   end subroutine
 
   !> time construct_tree and calc_velocity over a strided sample of targets

@@ -23,8 +23,9 @@ def available() -> bool:
 
 def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.1,
                 delta=5e-4, dt=1e-6, quant_circ=9.97e-4, corea=8.244023e-9,
-                tree_extra_correction=True, deriv_order="second", nsteps=1) -> str:
-    lines = [
+                tree_extra_correction=True, deriv_order="second", nsteps=1,
+                runge_kutta=False) -> str:
+    lines = [f"runge_kutta {'T' if runge_kutta else 'F'}",
         f"nsteps {nsteps}", "shots 1000000", "recon_shots 1000000", f"dt {dt!r}",
         f"delta {delta!r}", f"box_size {box_size!r}", f"velocity {velocity}",
         f"boundary {boundary}", f"quant_circ {quant_circ!r}", f"corea {corea!r}",
@@ -138,4 +139,37 @@ def closest_run(fil, *, pinsert=False, threads=1, timeout=600, **params) -> dict
     out["closest"] = take("<i4", n)
     out["closestd"] = take("<f8", n)
     assert off == len(raw)
+    return out
+
+
+def probe_run(fil, probes, *, motion=False, threads=1, timeout=600, **params) -> dict:
+    """Velocity at probe points the way mesh_velocity sums it
+    (timestep.f90:374-452): ghostp, construct_tree, [pmotion], then per probe
+    biot_savart_general(_shift) or tree_walk_general over the shells.
+    Returns the positions and ghosts the sums saw and u at the probes."""
+    if not available():
+        raise FileNotFoundError(DRIVER)
+    flags = 16 | (32 if motion else 0)
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    y = np.ascontiguousarray(probes, dtype="<f8")
+    with tempfile.TemporaryDirectory(prefix="qvref_") as wd:
+        with open(os.path.join(wd, "run.in"), "w") as fh:
+            fh.write(run_in_text(nsteps=1, **params))
+        os.makedirs(os.path.join(wd, "data"), exist_ok=True)
+        _write_state(os.path.join(wd, "state.bin"), fil, 1, flags, 0)
+        with open(os.path.join(wd, "probes.bin"), "wb") as fh:
+            np.array([y.shape[0]], dtype="<i4").tofile(fh)
+            y.tofile(fh)
+        proc = subprocess.run([DRIVER], cwd=wd, env=env, capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0 or "FATAL" in proc.stdout:
+            raise RuntimeError(f"reference driver failed: {proc.stdout[-2000:]}{proc.stderr[-2000:]}")
+        raw = open(os.path.join(wd, "probe.bin"), "rb").read()
+    n = int(np.frombuffer(raw, dtype="<i4", count=1)[0])
+    arr = np.frombuffer(raw, dtype="<f8", offset=4)
+    out = {"pcount": n, "x": arr[:3 * n].reshape(n, 3).copy(),
+           "ghosti": arr[3 * n:6 * n].reshape(n, 3).copy(),
+           "u": arr[6 * n:].reshape(-1, 3).copy()}
+    assert out["u"].shape == y.shape
     return out

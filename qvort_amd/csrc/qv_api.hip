@@ -99,6 +99,7 @@ int velocity_stage(Ctx &c) {
   TRY(hipMemsetAsync(c.counters, 0, 4 * sizeof(int32_t), c.stream));
   TRY(hipMemsetAsync(c.evals, 0, 4 * sizeof(int64_t), c.stream));
   TRY(launch_ghost_lia(c));
+  c.ghosts_valid = true;
   TRY(hipEventRecord(c.ev[1], c.stream));
   int rc = QV_OK;
   // construct_tree runs whenever tree_theta > 0 (run.f90:43-45), also with
@@ -310,7 +311,7 @@ int qv_destroy(qv_ctx *ctx) {
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
-                c->tree.linf, c->close_i, c->close_d};
+                c->tree.linf, c->close_i, c->close_d, c->probe_buf};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->h_evals) (void)hipHostFree(c->h_evals);
@@ -354,6 +355,7 @@ int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infro
   c->pcount = pcount;
   c->nact = nact;
   c->have_velocity = false;
+  c->ghosts_valid = false;
   c->links_consistent = consistent;
   TRY(hipMemcpyAsync(c->infront, infront, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
                      c->stream));
@@ -445,6 +447,19 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, doubl
   return QV_OK;
 }
 
+int qv_rk3_step(qv_ctx *ctx) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  for (int stage = 0; stage < 3; ++stage) {
+    const int rc = velocity_stage(*c);
+    if (rc) return rc;
+    TRY(launch_rk3_update(*c, stage));
+  }
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
 int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
   if (!ctx || !closest || !closestd) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -465,6 +480,40 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
   TRY(launch_closest(*c, tree));
   TRY(hipMemcpyAsync(closest, c->close_i, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   TRY(hipMemcpyAsync(closestd, c->close_d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, double *grad,
+                      int32_t flags) {
+  if (!ctx || nprobe < 0 || (nprobe > 0 && (!y || !u))) return QV_ERR_ARG;
+  if (flags & ~(QV_PROBE_GHOSTP | QV_PROBE_REFERENCE_SCALE)) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  const bool tree = c->p.velocity == QV_VEL_TREE;
+  if (tree && grad) return QV_ERR_ARG;
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  if (flags & QV_PROBE_GHOSTP) {
+    TRY(launch_ghost_lia(*c));
+    c->ghosts_valid = true;
+  }
+  if (tree ? !c->tree_valid : !c->ghosts_valid) return QV_ERR_STATE;
+  if (nprobe == 0) return QV_OK;
+  if (c->probe_cap < nprobe) {
+    TRY(hipStreamSynchronize(c->stream));
+    TRY(dalloc(c->probe_buf, (size_t)15 * nprobe));
+    c->probe_cap = nprobe;
+  }
+  double *d_y = c->probe_buf, *d_u = d_y + 3 * (size_t)nprobe, *d_g = d_u + 3 * (size_t)nprobe;
+  TRY(hipMemcpyAsync(d_y, y, 3 * (size_t)nprobe * sizeof(double), hipMemcpyHostToDevice,
+                     c->stream));
+  TRY(launch_probe(*c, nprobe, d_y, d_u, grad ? d_g : nullptr, tree,
+                   (flags & QV_PROBE_REFERENCE_SCALE) != 0));
+  TRY(hipMemcpyAsync(u, d_u, 3 * (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost,
+                     c->stream));
+  if (grad)
+    TRY(hipMemcpyAsync(grad, d_g, 9 * (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream));
   TRY(hipStreamSynchronize(c->stream));
   return QV_OK;
 }

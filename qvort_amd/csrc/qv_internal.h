@@ -87,6 +87,11 @@ struct Ctx {
   int32_t *close_i = nullptr;   // closest (1-based), per slot
   double *close_d = nullptr;    // closestd, per slot
   int32_t close_cap = 0;
+  // ghosts of the last ghostp (velocity stage or a probe call with
+  // QV_PROBE_GHOSTP); an upload invalidates them
+  bool ghosts_valid = false;
+  double *probe_buf = nullptr;  // probes (3n), u (3n), gradient (9n)
+  int32_t probe_cap = 0;
   // state (SoA), slot order
   Vec3 x, gi, gb, gii, gbb, u, uli, ubs, u1, u2, u3;
   int32_t *infront = nullptr, *behind = nullptr;
@@ -157,6 +162,7 @@ hipError_t launch_aos_to_soa(Ctx &c, const double *d_aos, Vec3 dst);
 hipError_t launch_soa_to_aos(Ctx &c, Vec3 src, double *d_aos);
 hipError_t launch_ghost_lia(Ctx &c);
 hipError_t launch_ab_update(Ctx &c);
+hipError_t launch_rk3_update(Ctx &c, int stage);
 hipError_t launch_enforce_periodic(Ctx &c);
 hipError_t launch_direct(Ctx &c);
 int tree_build(Ctx &c);          // returns qv_status
@@ -164,5 +170,7 @@ hipError_t launch_walk(Ctx &c);
 hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 hipError_t launch_closest(Ctx &c, bool tree);
+hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
+                        bool tree, bool ref_scale);
 
 }  // namespace qv

@@ -137,6 +137,38 @@ k_ab_update(int n, double dt, const int32_t *__restrict__ inf, double *__restric
   u1x[i] = u[0]; u1y[i] = u[1]; u1z[i] = u[2];
 }
 
+// the low-storage update of one pmotion_RK3 stage (timestep.f90:95-176):
+// stage 0: u1 = u dt, x += u1/3; stage 1: u1 = -(2/3) u1 + u dt, x += u1;
+// stage 2: u1 = -u1 + u dt, x += u1/2
+__global__ void __launch_bounds__(kBlock)
+k_rk3_update(int n, double dt, int stage, const int32_t *__restrict__ inf,
+             double *__restrict__ X, double *__restrict__ Y, double *__restrict__ Z,
+             const double *__restrict__ ux, const double *__restrict__ uy,
+             const double *__restrict__ uz, double *__restrict__ u1x, double *__restrict__ u1y,
+             double *__restrict__ u1z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || inf[i] == 0) return;
+  double *P[3] = {X, Y, Z}, *A[3] = {u1x, u1y, u1z};
+  const double *U[3] = {ux, uy, uz};
+  for (int k = 0; k < 3; ++k) {
+    double a = A[k][i];
+    const double v = U[k][i];
+    double x = P[k][i];
+    if (stage == 0) {
+      a = v * dt;
+      x = x + (a / 3.);
+    } else if (stage == 1) {
+      a = -((2. / 3.) * a) + v * dt;
+      x = x + a;
+    } else {
+      a = -a + v * dt;
+      x = x + 0.5 * a;
+    }
+    A[k][i] = a;
+    P[k][i] = x;
+  }
+}
+
 // enforce_periodic / _yz / _z wrap part (periodic.f90:188-276); loop removal
 // of the open axes is host topology and not done here
 __global__ void k_wrap(int n, double L, int k0, const int32_t *__restrict__ inf,
@@ -216,6 +248,13 @@ hipError_t launch_ab_update(Ctx &c) {
   hipLaunchKernelGGL(k_ab_update, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
                      c.p.dt, c.infront, c.x.x, c.x.y, c.x.z, c.u.x, c.u.y, c.u.z, c.u1.x, c.u1.y,
                      c.u1.z, c.u2.x, c.u2.y, c.u2.z, c.u3.x, c.u3.y, c.u3.z);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk3_update(Ctx &c, int stage) {
+  hipLaunchKernelGGL(k_rk3_update, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
+                     c.p.dt, stage, c.infront, c.x.x, c.x.y, c.x.z, c.u.x, c.u.y, c.u.z, c.u1.x,
+                     c.u1.y, c.u1.z);
   return hipGetLastError();
 }
 

@@ -1,0 +1,245 @@
+// qv_probe.hip -- superfluid velocity (and its gradient) at probe points.
+//
+// The sums of mesh_velocity (timestep.f90:374-452) and of the quasiparticle
+// Hamiltonian (velocity_quasip, hamiltonian.f90:26-97): the central shell,
+// then the image shells of the boundary in the order p outer .. r inner, one
+// running sum per probe.
+//   velocity BS:   biot_savart_general / _shift (timestep.f90:516-563) over
+//                  the current positions with the context's ghosts, in the
+//                  reference's operation order (bit-identical sums).  The
+//                  optional gradient du_k/dy_m is the analytic derivative of
+//                  the same terms with respect to the probe position y; it
+//                  replaces the six extra evaluations and central
+//                  differences of velocity_quasip (hamiltonian.f90:64-79).
+//   velocity Tree: tree_walk_general (tree.f90:425-488) over the tree of the
+//                  last velocity stage, one thread per probe, stackless DFS
+//                  on skip pointers.  The reference adds leaves as
+//                  u_bs/4/pi (no Gamma) and cells without Gamma/(4 pi); by
+//                  default both get Gamma/(4 pi) here, and
+//                  QV_PROBE_REFERENCE_SCALE keeps the reference's factors.
+#include "qv_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace qv {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 256;
+
+struct ProbeArgs {
+  int n, nprobe;
+  double qc, theta;
+  int grad, ref_scale;
+  const double *X, *Y, *Z, *GX, *GY, *GZ;
+  const int32_t *inf;
+  const double *py;  // probes, (nprobe,3) row-major
+  double *u;         // (nprobe,3)
+  double *g;         // (nprobe,9): g[9q + 3k + m] = du_k/dy_m
+  // tree
+  int64_t nnodes;
+  const NodeRec *rec;
+  const double2 *geo;
+  const int32_t *count;
+  const double *mp;  // internal: circ(3), C(9) row-major C[i][k]; leaf: ghosti(3)
+};
+
+// one term of biot_savart_general(_shift) at probe y, source x (+ shift s),
+// ghost g; adds its derivative with respect to y when G != nullptr
+template <bool kGrad>
+__device__ inline void general_term(const double y[3], const double xj[3], const double gj[3],
+                                    const double s[3], bool shifted, double qc, double u[3],
+                                    double G[9]) {
+  double xs[3], r[3], e[3];
+  for (int k = 0; k < 3; ++k) xs[k] = shifted ? xj[k] + s[k] : xj[k];
+  // dist_gen_sq(y, xs) (general.f90:142-148)
+  const double a = ((y[0] - xs[0]) * (y[0] - xs[0]) + (y[1] - xs[1]) * (y[1] - xs[1])) +
+                   (y[2] - xs[2]) * (y[2] - xs[2]);
+  for (int k = 0; k < 3; ++k) r[k] = xs[k] - y[k];
+  for (int k = 0; k < 3; ++k) e[k] = gj[k] - xj[k];
+  const double b = 2. * ((r[0] * e[0] + r[1] * e[1]) + r[2] * e[2]);
+  const double c = ((gj[0] - xj[0]) * (gj[0] - xj[0]) + (gj[1] - xj[1]) * (gj[1] - xj[1])) +
+                   (gj[2] - xj[2]) * (gj[2] - xj[2]);
+  const double D = (4. * a) * c - b * b;
+  if (D == 0.) return;
+  const double den = (2. * kPi) * D;
+  const double v[3] = {r[1] * e[2] - r[2] * e[1], r[2] * e[0] - r[0] * e[2],
+                       r[0] * e[1] - r[1] * e[0]};
+  const double sa = sqrt(a), s2 = sqrt((a + b) + c);
+  const double F = (2. * c + b) / s2 - b / sa;
+  for (int k = 0; k < 3; ++k) u[k] = u[k] + ((v[k] * qc) / den) * F;
+  if constexpr (kGrad) {
+    // u = (qc/(2 pi)) v F / D with da/dy = -2r, db/dy = -2e, dc/dy = 0,
+    // dv_k/dy_m = -eps_kmn e_n
+    const double w = r[0] + e[0], w1 = r[1] + e[1], w2 = r[2] + e[2];
+    const double ww[3] = {w, w1, w2};
+    const double s23 = s2 * s2 * s2, sa3 = sa * sa * sa;
+    const double h = qc / (2. * kPi);
+    double dF[3], dD[3];
+    for (int m = 0; m < 3; ++m) {
+      dF[m] = -2. * e[m] / s2 + (2. * c + b) * ww[m] / s23 + 2. * e[m] / sa - b * r[m] / sa3;
+      dD[m] = -8. * c * r[m] + 4. * b * e[m];
+    }
+    const double iD = 1. / D;
+    for (int k = 0; k < 3; ++k) {
+      for (int m = 0; m < 3; ++m) {
+        double dv = 0.;  // -eps_kmn e_n
+        if (k != m) {
+          const int nn = 3 - k - m;
+          const double sgn = ((k + 1) % 3 == m) ? -1. : 1.;  // eps_kmn = +1 for cyclic (k,m,n)
+          dv = sgn * e[nn];
+        }
+        G[3 * k + m] += h * iD * (dv * F + v[k] * dF[m] - v[k] * F * dD[m] * iD);
+      }
+    }
+  }
+}
+
+template <bool kGrad>
+__global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile], gx[kTile], gy[kTile], gz[kTile];
+  __shared__ int32_t sok[kTile];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = q < A.nprobe;
+  double y[3] = {0., 0., 0.};
+  if (act)
+    for (int k = 0; k < 3; ++k) y[k] = A.py[3 * q + k];
+  double u[3] = {0., 0., 0.};
+  double G[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int sh = 0; sh < S.n; ++sh) {
+    const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
+    for (int base = 0; base < A.n; base += kTile) {
+      __syncthreads();
+      const int j = base + threadIdx.x;
+      const bool in = j < A.n && A.inf[j] != 0;
+      sx[threadIdx.x] = in ? A.X[j] : 0.;
+      sy[threadIdx.x] = in ? A.Y[j] : 0.;
+      sz[threadIdx.x] = in ? A.Z[j] : 0.;
+      gx[threadIdx.x] = in ? A.GX[j] : 0.;
+      gy[threadIdx.x] = in ? A.GY[j] : 0.;
+      gz[threadIdx.x] = in ? A.GZ[j] : 0.;
+      sok[threadIdx.x] = in;
+      __syncthreads();
+      const int m = act ? min(kTile, A.n - base) : 0;
+      for (int jj = 0; jj < m; ++jj) {
+        if (!sok[jj]) continue;
+        const double xj[3] = {sx[jj], sy[jj], sz[jj]};
+        const double gj[3] = {gx[jj], gy[jj], gz[jj]};
+        general_term<kGrad>(y, xj, gj, s, sh != 0, A.qc, u, G);
+      }
+    }
+  }
+  if (!act) return;
+  for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
+  if constexpr (kGrad)
+    for (int k = 0; k < 9; ++k) A.g[9 * q + k] = G[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.nprobe) return;
+  const double y[3] = {A.py[3 * q], A.py[3 * q + 1], A.py[3 * q + 2]};
+  double u[3] = {0., 0., 0.};
+  for (int sh = 0; sh < S.n; ++sh) {
+    const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
+    int64_t n = 0;
+    while (n < A.nnodes) {
+      const NodeRec &r = A.rec[n];
+      const double c3[3] = {r.cx, r.cy, r.cz};
+      const double d0 = y[0] - (c3[0] + s[0]), d1 = y[1] - (c3[1] + s[1]),
+                   d2 = y[2] - (c3[2] + s[2]);
+      const double dist = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+      const int32_t cnt = A.count[n];
+      if (dist < 2.220446049250313e-16) {  // epsilon(0.): avoid 1/0 (tree.f90:441)
+        n = r.skip;
+        continue;
+      }
+      const double theta = A.geo[n].x / dist;
+      if (cnt == 1) {
+        // circ of a one-point cell is (0 + ghosti) - x (tree.f90:160-165)
+        double R1[3], R2[3];
+        for (int k = 0; k < 3; ++k) {
+          const double circ = r.g[k] - c3[k];
+          R1[k] = (c3[k] + s[k]) - y[k];
+          R2[k] = ((c3[k] + circ) + s[k]) - y[k];
+        }
+        const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
+        const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
+        const double cc = (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
+        const double f = (a + b) / cc;
+        const double ub[3] = {f * (R1[1] * R2[2] - R1[2] * R2[1]),
+                              f * (R1[2] * R2[0] - R1[0] * R2[2]),
+                              f * (R1[0] * R2[1] - R1[1] * R2[0])};
+        for (int k = 0; k < 3; ++k)
+          u[k] = u[k] + (A.ref_scale ? (ub[k] / 4.) / kPi : ((ub[k] * A.qc) / 4.) / kPi);
+        n = r.skip;
+      } else if (theta < A.theta) {
+        const double *mp = A.mp + 12 * n;
+        const double *circ = mp;
+        const double *C = mp + 3;  // C[3*i + k] = Cij(i+1, k+1)
+        const double v[3] = {d0, d1, d2};
+        double ub[3] = {circ[1] * v[2] - circ[2] * v[1], circ[2] * v[0] - circ[0] * v[2],
+                        circ[0] * v[1] - circ[1] * v[0]};
+        const double d3 = (dist * dist) * dist;
+        const double d5 = (((dist * dist) * dist) * dist) * dist;
+        for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
+        for (int k = 0; k < 3; ++k) {
+          for (int l = 0; l < 3; ++l) {
+            const double dcl = (v[0] * C[l] + v[1] * C[3 + l]) + v[2] * C[6 + l];
+            for (int m = 0; m < 3; ++m) {
+              double e = 0.;
+              if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
+              ub[m] = ub[m] - (e * C[3 * k + l]) / d3;
+            }
+            for (int m = 0; m < 3; ++m) {
+              double e = 0.;
+              if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
+              ub[m] = ub[m] + (((3. * e) * v[k]) * dcl) / d5;
+            }
+          }
+        }
+        for (int k = 0; k < 3; ++k)
+          u[k] = A.ref_scale ? u[k] + ub[k] : u[k] + ((ub[k] * A.qc) / 4.) / kPi;
+        n = r.skip;
+      } else {
+        n = n + 1;
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
+}
+
+}  // namespace
+
+hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
+                        bool tree, bool ref_scale) {
+  ProbeArgs A;
+  A.n = c.pcount;
+  A.nprobe = nprobe;
+  A.qc = c.p.quant_circ;
+  A.theta = c.p.tree_theta;
+  A.grad = d_g != nullptr;
+  A.ref_scale = ref_scale;
+  A.X = c.x.x; A.Y = c.x.y; A.Z = c.x.z;
+  A.GX = c.gi.x; A.GY = c.gi.y; A.GZ = c.gi.z;
+  A.inf = c.infront;
+  A.py = d_y;
+  A.u = d_u;
+  A.g = d_g;
+  A.nnodes = tree ? c.tree.nnodes : 0;
+  A.rec = c.tree.rec;
+  A.geo = c.tree.geo;
+  A.count = c.tree.count;
+  A.mp = c.tree.mp;
+  const Shells S = make_shells(c.p);
+  const dim3 g((nprobe + kBlock - 1) / kBlock);
+  if (tree)
+    hipLaunchKernelGGL(k_probe_tree, g, dim3(kBlock), 0, c.stream, A, S);
+  else if (d_g)
+    hipLaunchKernelGGL(k_probe_bs<true>, g, dim3(kBlock), 0, c.stream, A, S);
+  else
+    hipLaunchKernelGGL(k_probe_bs<false>, g, dim3(kBlock), 0, c.stream, A, S);
+  return hipGetLastError();
+}
+
+}  // namespace qv

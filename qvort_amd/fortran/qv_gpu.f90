@@ -22,6 +22,7 @@ module qv_gpu
   integer(c_int32_t), parameter, public :: QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, &
                                            QV_BC_OPENX = 2, QV_BC_OPENXY = 3
   integer(c_int32_t), parameter, public :: QV_FP64 = 0, QV_FP32C = 1
+  integer(c_int32_t), parameter, public :: QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2
   integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
                                        QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
                                        QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6
@@ -39,9 +40,9 @@ module qv_gpu
   end type
 
   public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
-            qv_velocity, qv_ab_step, qv_enforce_periodic, qv_step, qv_download, &
-            qv_tree_dump, qv_closest, qv_vjp, qv_jvp, qv_get_timings, qv_status_string, &
-            qv_check
+            qv_velocity, qv_ab_step, qv_rk3_step, qv_enforce_periodic, qv_step, qv_download, &
+            qv_tree_dump, qv_closest, qv_probe_velocity, qv_vjp, qv_jvp, qv_get_timings, &
+            qv_status_string, qv_check
 
   interface
     integer(c_int) function qv_create(params, ctx) bind(C, name='qv_create')
@@ -77,6 +78,10 @@ module qv_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function qv_rk3_step(ctx) bind(C, name='qv_rk3_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
     integer(c_int) function qv_enforce_periodic(ctx) bind(C, name='qv_enforce_periodic')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -101,6 +106,19 @@ module qv_gpu
       type(c_ptr), value :: ctx
       integer(c_int32_t), intent(inout) :: closest(*)
       real(c_double), intent(inout) :: closestd(*)
+    end function
+    !> u (and, if grad is not c_null_ptr, du_k/dy_m in grad(3,3,n) with
+    !> grad(m,k,q) = du_k/dy_m) at probe points y(3,n): mesh_velocity's sums,
+    !> and the gradient velocity_quasip takes by central differences
+    integer(c_int) function qv_probe_velocity(ctx, nprobe, y, u, grad, flags) &
+        bind(C, name='qv_probe_velocity')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nprobe
+      real(c_double), intent(in) :: y(3, *)
+      real(c_double), intent(out) :: u(3, *)
+      type(c_ptr), value :: grad
+      integer(c_int32_t), value :: flags
     end function
     integer(c_int) function qv_vjp(ctx, w, xbar) bind(C, name='qv_vjp')
       import :: c_int, c_ptr, c_double

@@ -4,13 +4,14 @@
 !> does, unknown labels ignored) and its f(:) arrays in the host layout
 !> x(3,pcount), infront, behind (cdata.f90:20-41).  The time loop follows
 !> run.f90:36-169 for the hot path: velocity (ghostp + construct_tree +
-!> calc_velocity), the AB ladder of pmotion, the periodic wrap, t = t + dt,
+!> calc_velocity), the AB ladder of pmotion (or pmotion_RK3 with
+!> runge_kutta T, run.f90:50-54), the periodic wrap, t = t + dt,
 !> and every `shots` steps the ts.log line and the var%04d.log snapshot
 !> (output.f90:55-143).  Line operations (pinsert, precon, premove) are not
 !> part of this driver.
 !>
 !> run.in labels used: nsteps shots dt delta box_size velocity boundary
-!> quant_circ corea tree_theta tree_extra_correction deriv_order initf pcount
+!> quant_circ corea tree_theta tree_extra_correction deriv_order runge_kutta initf pcount
 !> and gpu_precision (fp64 | fp32c, this driver's own).
 !> initf: single_loop (initial_loop.f90:11-39) or state_file (state.bin:
 !> int32 version, pcount, nsteps, flags, nsample; real64 x(3,pcount);
@@ -29,7 +30,7 @@ program qv_run
   real(c_double) :: dt = 1e-6_c_double, delta = 5e-4_c_double, box_size = 0._c_double
   real(c_double) :: quant_circ = 9.97e-4_c_double, corea = 8.244023e-9_c_double
   real(c_double) :: tree_theta = 0._c_double
-  logical :: tree_extra_correction = .true.
+  logical :: tree_extra_correction = .true., runge_kutta = .false.
   character(len=30) :: velocity = 'Tree', boundary = 'periodic', deriv_order = 'second'
   character(len=30) :: initf = 'single_loop', gpu_precision = 'fp64'
   ! state
@@ -84,9 +85,13 @@ program qv_run
         trim(velocity), ', precision ', trim(gpu_precision)
 
   do itime = 1, nsteps
-    call qv_check(qv_velocity(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(evals)), &
-                  'qv_velocity')
-    call qv_check(qv_ab_step(ctx), 'qv_ab_step')
+    if (runge_kutta) then
+      call qv_check(qv_rk3_step(ctx), 'qv_rk3_step')
+    else
+      call qv_check(qv_velocity(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(evals)), &
+                    'qv_velocity')
+      call qv_check(qv_ab_step(ctx), 'qv_ab_step')
+    end if
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
     t = t + dt
     if (mod(itime, shots) == 0) then
@@ -154,6 +159,8 @@ contains
           read(buffer, *, iostat=ios) init_pcount
         case ('gpu_precision')
           read(buffer, *, iostat=ios) gpu_precision
+        case ('runge_kutta')
+          read(buffer, *, iostat=ios) runge_kutta
       end select
       ios = 0
     end do

@@ -28,10 +28,11 @@ PREC = {"fp64": 0, "fp32c": 1}
 
 QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
     QV_ERR_NO_DEVICE = range(7)
+QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE = 1, 2
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
-           "qv_velocity", "qv_ab_step", "qv_enforce_periodic", "qv_step", "qv_download",
-           "qv_tree_dump", "qv_closest", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
+           "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic", "qv_step", "qv_download",
+           "qv_tree_dump", "qv_closest", "qv_probe_velocity", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
            "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_batch_create",
            "qv_batch_destroy", "qv_batch_upload", "qv_batch_velocity", "qv_batch_step",
            "qv_batch_download", "qv_batch_tune", "qv_batch_set_variant", "qv_batch_info",
@@ -105,12 +106,14 @@ def lib():
         L.qv_upload_history.argtypes = [vp, d, d, d]
         L.qv_velocity.argtypes = [vp, d, d, d, i64]
         L.qv_ab_step.argtypes = [vp]
+        L.qv_rk3_step.argtypes = [vp]
         L.qv_enforce_periodic.argtypes = [vp]
         L.qv_step.argtypes = [vp, C.c_int32]
         L.qv_download.argtypes = [vp, d, d, d, d, d]
         L.qv_tree_dump.argtypes = [vp, d, C.c_int64, i64]
         L.qv_get_timings.argtypes = [vp, P(QvTimings)]
         L.qv_closest.argtypes = [vp, i32, d]
+        L.qv_probe_velocity.argtypes = [vp, C.c_int32, d, d, d, C.c_int32]
         L.qv_vjp.argtypes = [vp, d, d]
         L.qv_jvp.argtypes = [vp, d, d]
         L.qv_set_stream.argtypes = [vp, vp]
@@ -218,6 +221,10 @@ class Context:
     def ab_step(self) -> None:
         _check(lib().qv_ab_step(self._h), "qv_ab_step")
 
+    def rk3_step(self) -> None:
+        """pmotion_RK3: three velocity stages with the low-storage update."""
+        _check(lib().qv_rk3_step(self._h), "qv_rk3_step")
+
     def enforce_periodic(self) -> None:
         _check(lib().qv_enforce_periodic(self._h), "qv_enforce_periodic")
 
@@ -251,6 +258,22 @@ class Context:
             raise ValueError("closest/closestd must have one entry per slot")
         _check(lib().qv_closest(self._h, _ip(cl), _dp(cd)), "qv_closest")
         return cl, cd
+
+    def probe_velocity(self, y, grad: bool = False, ghostp: bool = False,
+                       reference_scale: bool = False):
+        """Superfluid velocity at probe points y (n,3), as mesh_velocity sums
+        it; with grad=True also du_k/dy_m as an (n,3,3) array (BS only)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        n = y.shape[0]
+        if y.shape != (n, 3):
+            raise ValueError("probes must be (n,3)")
+        u = np.zeros((n, 3))
+        g = np.zeros((n, 3, 3)) if grad else None
+        flags = (QV_PROBE_GHOSTP if ghostp else 0) | \
+            (QV_PROBE_REFERENCE_SCALE if reference_scale else 0)
+        _check(lib().qv_probe_velocity(self._h, n, _dp(y), _dp(u), _dp(g), flags),
+               "qv_probe_velocity")
+        return (u, g) if grad else u
 
     def vjp(self, w) -> np.ndarray:
         """xbar = (dU/dx)^T w at the device positions (direct-sum velocity)."""

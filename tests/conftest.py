@@ -26,6 +26,16 @@ def closest_names():
         return sorted(json.load(fh))
 
 
+def probe_names():
+    with open(os.path.join(GOLDEN, "probe_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
+def rk3_names():
+    with open(os.path.join(GOLDEN, "rk3_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
 def load_golden(name):
     z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     d = {k: z[k] for k in z.files}
@@ -49,6 +59,7 @@ class Fil:
 def oracle_params(p):
     from oracle import port
     kw = dict(p)
+    kw.pop("runge_kutta", None)
     kw["deriv_order"] = 4 if kw.get("deriv_order") == "fourth" else 2
     return port.make_params(**kw)
 
@@ -56,6 +67,7 @@ def oracle_params(p):
 def gpu_params(p, precision="fp64"):
     from qvort_amd import gpu
     kw = dict(p)
+    kw.pop("runge_kutta", None)
     kw["deriv_order"] = 4 if kw.get("deriv_order") == "fourth" else 2
     return gpu.params(precision=precision, **kw)
 

@@ -7,12 +7,24 @@ case records the run.in parameters it used.  Run from the repository root:
 
     python tests/golden/make_golden.py              # velocity / tree / AB cases
     python tests/golden/make_golden.py --closest    # neighbour-search cases
+    python tests/golden/make_golden.py --probe      # velocity at probe points
+    python tests/golden/make_golden.py --rk3        # pmotion_RK3 trajectories
 
 The neighbour-search cases (closest_*.npz, index in closest_index.json) run
 one step of ghostp, construct_tree, pmotion, optionally pinsert, then
 pclose_tree or pclose, and store the build state (x0, infront0, behind0),
 the state the search saw (x, infront, behind), closest/closestd before the
 search (closest_in, closestd_in) and after it.
+
+The probe cases (probe_*.npz, index in probe_index.json) run ghostp,
+construct_tree, optionally pmotion, then sum the velocity at probe points as
+mesh_velocity does, and store the input state, the probes, the positions and
+ghosts the sums saw (x, ghosti) and u.
+
+The Runge-Kutta cases (rk3_*.npz, index in rk3_index.json) run the main loop
+with runge_kutta T (pmotion_RK3, then the wrap) and store the input, the
+positions and the Runge-Kutta storage u1 after nsteps, and the velocity of
+the first step's last stage.
 """
 from __future__ import annotations
 
@@ -103,7 +115,76 @@ def main_closest():
         json.dump(index, fh, indent=1, sort_keys=True)
 
 
+def probe_cases():
+    loops = G.random_loops(0.1, 5e-4, n_loops=4, seed=41)
+    loops_open = G.random_loops(0.1, 5e-4, n_loops=4, seed=42, wrap=False)
+    rng = np.random.Generator(np.random.PCG64(43))
+    y = rng.uniform(-0.05, 0.05, size=(300, 3))
+    base = dict(box_size=0.1, delta=5e-4, dt=1e-5, corea=1.3e-8, quant_circ=9.97e-4)
+    return [
+        ("probe_bs_periodic_motion", loops, y, dict(base, velocity="BS", tree_theta=0.0,
+                                                    boundary="periodic"), True),
+        ("probe_bs_openx", loops, y, dict(base, velocity="BS", tree_theta=0.0,
+                                          boundary="openx"), False),
+        ("probe_tree_periodic", loops, y, dict(base, velocity="Tree", tree_theta=0.4,
+                                               boundary="periodic"), False),
+        ("probe_tree_open_motion", loops_open, y, dict(base, velocity="Tree", tree_theta=0.6,
+                                                       boundary="open"), True),
+    ]
+
+
+def main_probe():
+    index = {}
+    for name, fil, y, params, motion in probe_cases():
+        r = refrun.probe_run(fil, y, motion=motion, **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind, probes=y, x=r["x"],
+                      ghosti=r["ghosti"], u=r["u"])
+        meta = dict(params=params, motion=motion, pcount=fil.pcount, nprobe=int(y.shape[0]))
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "|u|max", float(np.abs(r["u"]).max()), flush=True)
+    with open(os.path.join(OUT, "probe_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
+def rk3_cases():
+    ring = G.single_ring(256, 0.03)
+    loops = G.random_loops(0.1, 5e-4, n_loops=4, seed=61)
+    holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=3, seed=62), 30, seed=6)
+    base = dict(delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4, runge_kutta=True)
+    return [
+        ("rk3_ring_bs_open", ring, dict(base, velocity="BS", tree_theta=0.0, boundary="open",
+                                        box_size=0.2, delta=8e-4, dt=1e-5), 3),
+        ("rk3_loops_tree_periodic", loops, dict(base, velocity="Tree", tree_theta=0.4,
+                                                boundary="periodic", box_size=0.1), 3),
+        ("rk3_holes_bs_openx", holes, dict(base, velocity="BS", tree_theta=0.0,
+                                           boundary="openx", box_size=0.1), 2),
+    ]
+
+
+def main_rk3():
+    index = {}
+    for name, fil, params, nsteps in rk3_cases():
+        r = refrun.run(fil, nsteps=nsteps, threads=1, **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind, u_first=r["u"],
+                      x_final=r["x"], u1_final=r["u1"])
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount)
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, flush=True)
+    with open(os.path.join(OUT, "rk3_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def main():
+    if "--rk3" in sys.argv[1:]:
+        main_rk3()
+        return
+    if "--probe" in sys.argv[1:]:
+        main_probe()
+        return
     if "--closest" in sys.argv[1:]:
         main_closest()
         return

@@ -1,0 +1,149 @@
+"""Velocity at probe points on the GPU (qv_probe_velocity).
+
+BS sums bit for bit against the oracle on the same inputs (positions as the
+device holds them after a step, ghosts of the step's ghostp) and within
+1e-12 of the reference fixtures; Tree sums within 1e-12 relative of the
+oracle on the same tree (the device sums moments in another order).  The
+analytic gradient against Richardson-extrapolated central differences of the
+oracle (1e-6 relative) and against div u = 0, which holds for every
+straight segment's field.
+"""
+import numpy as np
+import pytest
+
+from conftest import Fil, gpu_params, load_golden, probe_names
+
+pytestmark = pytest.mark.gpu
+
+NAMES = probe_names()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _rel(a, b):
+    return np.max(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_probe_vs_golden(name):
+    from qvort_amd import gpu
+    g = load_golden(name)
+    meta = g["meta"]
+    _say("case", name)
+    with gpu.Context(gpu_params(meta["params"])) as c:
+        c.upload(g["x0"], g["infront"], g["behind"])
+        c.velocity()
+        if meta["motion"]:
+            c.ab_step()
+        u = c.probe_velocity(g["probes"], reference_scale=True)
+    assert _rel(u, g["u"]) <= 1e-12
+
+
+@pytest.mark.parametrize("boundary", ["periodic", "openx", "openxy", "open"])
+def test_bs_probe_bitexact_vs_oracle(boundary):
+    from oracle import port
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, n_loops=6, seed=51, wrap=boundary != "open")
+    kw = dict(velocity="BS", tree_theta=0.0, boundary=boundary, box_size=0.1, corea=1.3e-8,
+              dt=1e-5)
+    rng = np.random.Generator(np.random.PCG64(52))
+    y = rng.uniform(-0.05, 0.05, size=(1000, 3))
+    p = port.make_params(**kw)
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        c.ab_step()
+        x_now = c.download()[0]
+        u_stale = c.probe_velocity(y)
+        u_fresh = c.probe_velocity(y, ghostp=True)
+    gi0, _, _, _ = port.ghostp(p, fil)
+    cur = Fil(x_now, fil.infront, fil.behind)
+    gi1, _, _, _ = port.ghostp(p, cur)
+    assert np.array_equal(u_stale, port.probe_velocity(p, fil, cur, gi0, y, nthreads=16))
+    assert np.array_equal(u_fresh, port.probe_velocity(p, fil, cur, gi1, y, nthreads=16))
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.6])
+def test_tree_probe_vs_oracle(theta):
+    from oracle import port
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, target_points=20_000, seed=53)
+    kw = dict(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.1,
+              corea=1.3e-8)
+    rng = np.random.Generator(np.random.PCG64(54))
+    y = rng.uniform(-0.05, 0.05, size=(2000, 3))
+    p = port.make_params(**kw)
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u = c.probe_velocity(y)
+        ur = c.probe_velocity(y, reference_scale=True)
+    gi, _, _, _ = port.ghostp(p, fil)
+    ref = port.probe_velocity(p, fil, fil, gi, y, ref_scaling=False, nthreads=16)
+    refr = port.probe_velocity(p, fil, fil, gi, y, ref_scaling=True, nthreads=16)
+    _say("tree probe", theta, _rel(u, ref), _rel(ur, refr))
+    assert _rel(u, ref) <= 1e-12
+    assert _rel(ur, refr) <= 1e-12
+
+
+def test_gradient_vs_differences_and_divergence():
+    from oracle import port
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, n_loops=5, seed=55)
+    kw = dict(velocity="BS", tree_theta=0.0, boundary="periodic", box_size=0.1, corea=1.3e-8)
+    rng = np.random.Generator(np.random.PCG64(56))
+    y = rng.uniform(-0.05, 0.05, size=(64, 3))
+    p = port.make_params(**kw)
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u, G3 = c.probe_velocity(y, grad=True)
+    gi, _, _, _ = port.ghostp(p, fil)
+
+    def uo(yy):
+        return port.probe_velocity(p, fil, fil, gi, yy, nthreads=16)
+
+    assert np.array_equal(u, uo(y))
+    h = 1e-6
+    fd = np.zeros_like(G3)
+    for m in range(3):
+        d = []
+        for hh in (h, 2 * h):
+            e = np.zeros(3)
+            e[m] = hh
+            d.append((uo(y + e) - uo(y - e)) / (2 * hh))
+        fd[:, :, m] = (4 * d[0] - d[1]) / 3
+    scale = np.linalg.norm(G3.reshape(-1, 9), axis=1)
+    err = np.linalg.norm((G3 - fd).reshape(-1, 9), axis=1) / scale
+    _say("gradient rel err", err.max())
+    assert err.max() <= 1e-6
+    div = np.abs(np.trace(G3, axis1=1, axis2=2)) / scale
+    assert div.max() <= 1e-10
+
+
+def test_probe_state_and_argument_errors():
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, n_loops=2, seed=57)
+    y = np.zeros((4, 3))
+    with gpu.Context(gpu.params(velocity="BS")) as c:
+        c.upload_filaments(fil)
+        with pytest.raises(gpu.QvortError) as ei:
+            c.probe_velocity(y)
+        assert ei.value.status == gpu.QV_ERR_STATE
+        c.probe_velocity(y, ghostp=True)
+    with gpu.Context(gpu.params(velocity="Tree", tree_theta=0.4)) as c:
+        c.upload_filaments(fil)
+        with pytest.raises(gpu.QvortError) as ei:
+            c.probe_velocity(y)
+        assert ei.value.status == gpu.QV_ERR_STATE
+        c.velocity()
+        c.probe_velocity(y)
+        with pytest.raises(gpu.QvortError) as ei:
+            c.probe_velocity(y, grad=True)
+        assert ei.value.status == gpu.QV_ERR_ARG
