@@ -21,9 +21,19 @@
 // T < 0 take the full test.  The decisions are bit-identical to the
 // reference's.
 //
+// Per node step the record arrives with one s_load_dwordx16 (32-bit byte
+// offset), the lane's bit of the walk mask is taken with inverse_ballot (no
+// VALU), T < 0 is read from the sign of T's high word (a scalar test), and
+// the accepting lanes are M & ballot(d2 >= T).  In FP32C every internal
+// node's multipole is evaluated with weight 0 on the lanes that do not
+// accept: a uniform branch around it costs more (accumulator copies) than
+// the 7% of steps in which no lane accepts.
+//
 // QV_FP64: leaf terms in the reference's operation order, multipoles as
 // [circ x r - eps:C] / d^3 + 3 r x (C^T r) / d^5 (the reference's double
-// loop regrouped), the factor Gamma/(4 pi) applied once per sum.
+// loop regrouped) with 1/d from a refined reciprocal square root (a couple
+// of ulp instead of the correctly rounded sqrt and two divisions), the
+// factor Gamma/(4 pi) applied once per sum.
 // QV_FP32C: relative vectors in FP64, terms in FP32, per-target sums in FP64.
 #include "qv_internal.h"
 
@@ -96,12 +106,24 @@ __device__ inline void leaf64(bool on, const double xi[3], double cx, double cy,
 
 // cell term, tree.f90:400-411 (without Gamma/(4 pi)); r = x_i - (cent +
 // shift), d = |r|; added where `on`
+// 1/sqrt(x) to about an ulp: the hardware estimate (relative error below
+// 2^-22) refined by two Newton steps y <- y + y (1/2 - (x/2) y^2), each of
+// which squares the error
+__device__ inline double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = __builtin_fma(y, __builtin_fma(-(h * y), y, 0.5), y);
+  y = __builtin_fma(y, __builtin_fma(-(h * y), y, 0.5), y);
+  return y;
+}
+
 __device__ inline void cell64(bool on, const double r[3], double d2,
                               const double *__restrict__ m, double u[3]) {
-  const double d = sqrt(d2);
-  const double d3 = (d * d) * d;
-  const double d5 = (((d * d) * d) * d) * d;
-  const double id3 = on ? 1. / d3 : 0., id5 = on ? 1. / d5 : 0.;
+  // lanes that are off use d2 = 1 (finite terms) and weight 0
+  const double id = rsqrt64(on ? d2 : 1.);
+  const double id2 = id * id;
+  const double id3 = on ? id2 * id : 0.;
+  const double id5 = id3 * id2;
   const double c0 = m[0], c1 = m[1], c2 = m[2];
   const double *C = m + 3;  // C[3*i + k] = Cij(i,k)
   // eps(:,k,l) C(k,l): antisymmetric part
@@ -163,7 +185,8 @@ __device__ inline void cell32(bool on, const double rd[3], double d2, const floa
   const float id = __builtin_amdgcn_rsqf(on ? (float)d2 : 1.f);
   const float id2 = id * id;
   const float q = 3.f * id2;
-  const double id3 = (double)(on ? id2 * id : 0.f);
+  const float id3f = on ? id2 * id : 0.f;
+  const double id3 = (double)id3f;
   const float v0 = fmaf_(r2, C[6], fmaf_(r1, C[3], r0 * C[0]));
   const float v1 = fmaf_(r2, C[7], fmaf_(r1, C[4], r0 * C[1]));
   const float v2 = fmaf_(r2, C[8], fmaf_(r1, C[5], r0 * C[2]));
@@ -200,11 +223,19 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
   int n = 0, e = (int)A.nnodes;
   uint64_t M = active;
   int sp = 0;
+  typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+  const char *recb = reinterpret_cast<const char *>(A.rec);
+  const char *m32b = reinterpret_cast<const char *>(A.mp32);
   while (true) {
-    const NodeRec rec = A.rec[n];
+    // byte offset of record n in 32 bits (64 B per record, 16 floats of
+    // mp32 per node: the same offset); nnodes < 2^26
+    const uint32_t off = (uint32_t)n << 6;
+    const u32x16 raw = *reinterpret_cast<const u32x16 *>(recb + off);
+    NodeRec rec;
+    __builtin_memcpy(&rec, &raw, sizeof(NodeRec));
     ++wc.nstep;
     const int skip = uni(rec.skip), j = uni(rec.j);
-    const bool mine = (M >> lane) & 1ull;
+    const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
     wc.nvis += (uint32_t)__popcll(M);
     // r = x_i - (cent + shift), d2 as the reference forms dist (tree.f90:351-353)
     const double cs0 = kShift ? rec.cx + sv[0] : rec.cx;
@@ -218,9 +249,8 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
       const bool on = mine && j != t && j != bi && j != fi;
       const uint64_t L = __ballot(on);
       if (L) {
-        if (__ballot(on && d2 < 1e-30)) {
-          if (on && d2 < 1e-30 && sqrt(d2) < 2.220446049250313e-16) wc.singular = true;
-        }
+        // sqrt(d2) < epsilon(0.) (tree.f90:372) is exactly d2 < 2^-104
+        if (__ballot(on && d2 < 0x1p-104)) wc.singular = true;
         if (kMode == QV_FP64) {
           leaf64<kShift>(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
         } else {
@@ -230,19 +260,21 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
         wc.nleaf += (uint32_t)__popcll(L);
       }
     } else {
-      bool ok;
-      if (rec.T >= 0.) {
-        ok = mine && d2 >= rec.T;
+      // T < 0 (full test) read from the sign of T's high word, a scalar test
+      uint64_t Acc_;
+      if (uni((int32_t)(raw[7])) >= 0) {
+        Acc_ = M & __ballot(d2 >= rec.T);
       } else {
         const double2 g = A.geo[n];
-        ok = mine && accept_full(d2, g.x, g.y, A.theta, A.extra);
+        Acc_ = M & __ballot(accept_full(d2, g.x, g.y, A.theta, A.extra));
       }
-      const uint64_t Acc_ = __ballot(ok);
-      if (Acc_) {
-        if (kMode == QV_FP64) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
-        else cell32(ok, r, d2, rec.m6, A.mp32 + 16 * (int64_t)n, acc);
-        wc.ncell += (uint32_t)__popcll(Acc_);
+      const bool ok = __builtin_amdgcn_inverse_ballot_w64(Acc_);
+      if (kMode == QV_FP64) {
+        if (Acc_) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
+      } else {
+        cell32(ok, r, d2, rec.m6, reinterpret_cast<const float *>(m32b + off), acc);
       }
+      wc.ncell += (uint32_t)__popcll(Acc_);
       O = M & ~Acc_;
     }
     if (O != 0 && n + 1 != skip) {
@@ -268,8 +300,11 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
   }
 }
 
+// 8 wavefronts per SIMD: the register budget that leaves (48 VGPRs, 78
+// SGPRs, no spills in the node loop) hides more of the node loads' latency
+// than 7 waves with spare registers
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) k_walk(WalkArgs A, Shells S) {
+__global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
   // XCD-aware order: consecutive blocks of the target list go to one XCD
   const int G = gridDim.x;
