@@ -112,8 +112,9 @@ __device__ inline void pair32(float r0, float r1, float r2, float e0, float e1, 
   const float ab = a * b;
   const float dt = __builtin_fmaf(r2, s2, __builtin_fmaf(r1, s1, r0 * s0));
   const float g = (a + b) * (ia * ib);
-  const float f = dt >= 0.f ? g * __builtin_amdgcn_rcpf(ab + dt)
-                            : (g * (ab - dt)) * __builtin_amdgcn_rcpf(cc);
+  // one reciprocal of the selected denominator
+  const bool in = dt >= 0.f;
+  const float f = (in ? g : g * (ab - dt)) * __builtin_amdgcn_rcpf(in ? ab + dt : cc);
   const double fd = (double)f;
   acc.s0 = fma(fd, (double)c0, acc.s0);
   acc.s1 = fma(fd, (double)c1, acc.s1);
@@ -184,7 +185,7 @@ struct SrcChunk {
 // loop exits on a scalar compare, so the loop has no divergent exit and no
 // lane can be separated from the atomic.
 template <int kPer, int kSrc, bool kF64>
-__global__ void __launch_bounds__(64) k_batch_step(BatchArgs A, Shells S) {
+__global__ void __launch_bounds__(64, 4) k_batch_step(BatchArgs A, Shells S) {
   __shared__ SrcChunk<kSrc, kF64> t;
   const int lane = threadIdx.x;
   while (true) {
@@ -194,14 +195,17 @@ __global__ void __launch_bounds__(64) k_batch_step(BatchArgs A, Shells S) {
     const int off = A.sys_off[sys], end = off + A.sys_n[sys];
     int ti[kPer], bi[kPer];
     bool act[kPer];
-    double x[kPer][3], ul[kPer][3], uc[kPer][3], us[kPer][3];
-    AccF64 ac[kPer], as[kPer];
+    // cur: the sum of the shell being summed (a fixed accumulator per pair,
+    // no select between the central and the image sums); the central
+    // shell's is moved to uc / ac when that shell ends
+    double x[kPer][3], ul[kPer][3], uc[kPer][3], cur[kPer][3];
+    AccF64 ac[kPer], acur[kPer];
     for (int k = 0; k < kPer; ++k) {
       const int i = A.unit_first[unit] + lane + 64 * k;
       ti[k] = i;
       bi[k] = -1;
       act[k] = i < end && A.inf[i] != 0;
-      for (int c = 0; c < 3; ++c) x[k][c] = ul[k][c] = uc[k][c] = us[k][c] = 0.;
+      for (int c = 0; c < 3; ++c) x[k][c] = ul[k][c] = uc[k][c] = cur[k][c] = 0.;
       if (act[k]) {
         const int a = A.inf[i] - 1, b = A.beh[i] - 1;
         const int aa = A.inf[a] - 1, bb = A.beh[b] - 1;
@@ -250,8 +254,8 @@ __global__ void __launch_bounds__(64) k_batch_step(BatchArgs A, Shells S) {
             const double gj[3] = {t.gx[jj], t.gy[jj], t.gz[jj]};
             for (int k = 0; k < kPer; ++k) {
               if (!act[k] || j == bi[k] || (sh == 0 && j == ti[k])) continue;
-              if (sh == 0) pair64<false>(x[k], xj, gj, s, A.qc, uc[k]);
-              else pair64<true>(x[k], xj, gj, s, A.qc, us[k]);
+              if (sh == 0) pair64<false>(x[k], xj, gj, s, A.qc, cur[k]);
+              else pair64<true>(x[k], xj, gj, s, A.qc, cur[k]);
             }
           } else {
             const double sx = t.x[jj], sy = t.y[jj], sz = t.z[jj];
@@ -259,24 +263,55 @@ __global__ void __launch_bounds__(64) k_batch_step(BatchArgs A, Shells S) {
             for (int k = 0; k < kPer; ++k) {
               if (!act[k] || j == bi[k] || (sh == 0 && j == ti[k])) continue;
               pair32((float)(sx - xs[k][0]), (float)(sy - xs[k][1]), (float)(sz - xs[k][2]),
-                     ex, ey, ez, sh == 0 ? ac[k] : as[k]);
+                     ex, ey, ez, acur[k]);
             }
           }
+        }
+      }
+      if (sh == 0) {
+        for (int k = 0; k < kPer; ++k) {
+          for (int c = 0; c < 3; ++c) {
+            uc[k][c] = cur[k][c];
+            cur[k][c] = 0.;
+          }
+          ac[k] = acur[k];
+          acur[k] = AccF64();
         }
       }
     }
     for (int k = 0; k < kPer; ++k) {
       const int i = ti[k];
+      double us[3] = {cur[k][0], cur[k][1], cur[k][2]};  // the image shells' sum
       if (act[k]) {
         if constexpr (!kF64) {
+          // the two segments next to the target's own (infront(i) ->
+          // infront(infront(i)) and behind(behind(i)) -> behind(i)) in FP64:
+          // their FP32 terms (exact FP64 products) are taken back out of
+          // the central sum and the FP64 terms added
+          double adj[3] = {0., 0., 0.};
+          const int fi = A.inf[i] - 1, bb = A.beh[bi[k]] - 1;
+          const int js[2] = {fi, bb};
+          const double z3[3] = {0., 0., 0.};
+          for (int q = 0; q < (bb == fi ? 1 : 2); ++q) {
+            const int j = js[q];
+            if (j == i || j == bi[k]) continue;
+            const double xj[3] = {A.X[j], A.Y[j], A.Z[j]};
+            double gj[3];
+            ghost_of(A, A.inf[j] - 1, xj, gj);
+            AccF64 t;
+            pair32((float)(xj[0] - x[k][0]), (float)(xj[1] - x[k][1]), (float)(xj[2] - x[k][2]),
+                   (float)(gj[0] - xj[0]), (float)(gj[1] - xj[1]), (float)(gj[2] - xj[2]), t);
+            ac[k].s0 -= t.s0; ac[k].s1 -= t.s1; ac[k].s2 -= t.s2;
+            pair64<false>(x[k], xj, gj, z3, A.qc, adj);
+          }
           const double g4 = A.qc / (4. * kPi);
           ac[k].get(uc[k][0], uc[k][1], uc[k][2]);
-          as[k].get(us[k][0], us[k][1], us[k][2]);
-          for (int c = 0; c < 3; ++c) { uc[k][c] *= g4; us[k][c] *= g4; }
+          acur[k].get(us[0], us[1], us[2]);
+          for (int c = 0; c < 3; ++c) { uc[k][c] = uc[k][c] * g4 + adj[c]; us[c] *= g4; }
         }
         // u = u_LI + u_bs(central); u = u + u_bs(images)  (timestep.f90:225-240)
         double u0 = ul[k][0] + uc[k][0], u1 = ul[k][1] + uc[k][1], u2 = ul[k][2] + uc[k][2];
-        if (S.n > 1) { u0 = u0 + us[k][0]; u1 = u1 + us[k][1]; u2 = u2 + us[k][2]; }
+        if (S.n > 1) { u0 = u0 + us[0]; u1 = u1 + us[1]; u2 = u2 + us[2]; }
         A.ux[i] = u0; A.uy[i] = u1; A.uz[i] = u2;
       } else if (i < end) {
         A.ux[i] = A.uy[i] = A.uz[i] = 0.;
