@@ -16,6 +16,16 @@
 !>   pclose          line.f90:205-234     (flags bit 2, tree_theta = 0)
 !>   biot_savart_general(_shift) timestep.f90:516-563, tree_walk_general
 !>                   tree.f90:425-488     (flags bit 4, probe points)
+!>   the restart file data/var.dat: written after the run with the statements
+!>                   of data_dump (output.f90:272-285, flags bit 6), read before
+!>                   it with those of data_restore (initial.f90:500-529, flags
+!>                   bit 7; the run then continues at the restored itime + 1
+!>                   and t).  output.f90 and initial.f90 do not build here
+!>                   (itime clash, stiff_solver), so the statements are
+!>                   restated; the record layout is that of the reference's
+!>                   own type(qvort) under this compiler.
+!>   construct_tree's leaf print data/tree_mesh%03d.log (tree.f90:116-123)
+!>                   when run.in sets tree_print T and shots divides itime
 !> and mirrors the order of the main loop in run.f90:36-169 with the line
 !> operations left out, except in the neighbour-search mode (flags bit 2):
 !> one step of ghostp, construct_tree, pmotion, [pinsert], then the search,
@@ -56,7 +66,7 @@ program qv_ref_driver
   real, allocatable :: u_first(:,:), uli_first(:,:), ubs_first(:,:), gi_first(:,:), gb_first(:,:)
   real :: uvel(3)
   real(8) :: t0, t1, t2
-  integer :: nodes_written
+  integer :: nodes_written, dummy_itime
 
   call read_run_file()
   call set_boundary_flags()
@@ -69,6 +79,8 @@ program qv_ref_driver
     f(i)%ghosti = 0.; f(i)%ghostb = 0.; f(i)%ghostii = 0.; f(i)%ghostbb = 0.
     f(i)%u_s_LI = 0.; f(i)%u_s_BS = 0.; f(i)%u_sup = 0.; f(i)%u_mf = 0.
     f(i)%closest = 0; f(i)%closestd = 0.
+    f(i)%x_interp = 0.; f(i)%acc = 0.; f(i)%f_mf = 0.; f(i)%stretch = 0.; f(i)%curv = 0.
+    f(i)%closestd_loop = 0.; f(i)%delta = 0.; f(i)%t_recon = 0.
   end do
   do i = 1, pcount
     read(41) f(i)%x
@@ -90,6 +102,25 @@ program qv_ref_driver
   end do
   close(41)
 
+  nstart = 1
+  t = 0.
+  if (btest(flags, 7)) then
+    ! data_restore's reads (initial.f90:506-520); no quasi particles or
+    ! particles are written by this driver
+    deallocate(f)
+    open(unit=63, file='./data/var.dat', form='unformatted', status='old', action='read')
+    read(63) pcount
+    read(63) recon_count
+    read(63) dummy_itime
+    read(63) t
+    allocate(f(pcount))
+    read(63) f
+    read(63) quasi_pcount
+    read(63) part_count
+    close(63)
+    nstart = dummy_itime + 1
+  end if
+
   if (btest(flags, 1)) then
     call timing_run()
     stop
@@ -106,11 +137,10 @@ program qv_ref_driver
   allocate(u_first(3,pcount), uli_first(3,pcount), ubs_first(3,pcount))
   allocate(gi_first(3,pcount), gb_first(3,pcount))
   eval_first = 0
-  t = 0.
-  do itime = 1, nsteps_run
+  do itime = nstart, nstart + nsteps_run - 1
     call ghostp
     if (tree_theta > 0) call construct_tree
-    if (itime == 1 .and. btest(flags, 0) .and. tree_theta > 0) then
+    if (itime == nstart .and. btest(flags, 0) .and. tree_theta > 0) then
       open(unit=42, file='tree.bin', access='stream', form='unformatted', status='replace')
       nodes_written = 0
       call dump_node(vtree, 0)
@@ -121,7 +151,7 @@ program qv_ref_driver
     else
       call pmotion
     end if
-    if (itime == 1) then
+    if (itime == nstart) then
       do i = 1, pcount
         u_first(:,i) = f(i)%u
         uli_first(:,i) = f(i)%u_s_LI
@@ -147,6 +177,22 @@ program qv_ref_driver
       nullify(vtree)
     end if
   end do
+  itime = nstart + nsteps_run - 1
+
+  if (btest(flags, 6)) then
+    ! data_dump's writes (output.f90:274-283)
+    quasi_pcount = 0
+    part_count = 0
+    open(unit=53, file='./data/var.dat', form='unformatted', status='replace')
+    write(53) pcount
+    write(53) recon_count
+    write(53) itime
+    write(53) t
+    write(53) f
+    write(53) quasi_pcount
+    write(53) part_count
+    close(53)
+  end if
 
   open(unit=43, file='out.bin', access='stream', form='unformatted', status='replace')
   write(43) pcount

@@ -24,9 +24,10 @@ def available() -> bool:
 def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.1,
                 delta=5e-4, dt=1e-6, quant_circ=9.97e-4, corea=8.244023e-9,
                 tree_extra_correction=True, deriv_order="second", nsteps=1,
-                runge_kutta=False) -> str:
+                runge_kutta=False, shots=1000000, tree_print=False) -> str:
     lines = [f"runge_kutta {'T' if runge_kutta else 'F'}",
-        f"nsteps {nsteps}", "shots 1000000", "recon_shots 1000000", f"dt {dt!r}",
+        f"nsteps {nsteps}", f"shots {shots}", "recon_shots 1000000", f"dt {dt!r}",
+        f"tree_print {'T' if tree_print else 'F'}",
         f"delta {delta!r}", f"box_size {box_size!r}", f"velocity {velocity}",
         f"boundary {boundary}", f"quant_circ {quant_circ!r}", f"corea {corea!r}",
         f"tree_theta {tree_theta!r}",
@@ -49,11 +50,20 @@ def _write_state(path, fil, nsteps, flags, nsample, u1=None, u2=None, u3=None):
 
 
 def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None,
-        u1=None, u2=None, u3=None, timeout=3600, **params) -> dict:
-    """Run the reference driver; returns a dict of numpy arrays."""
+        u1=None, u2=None, u3=None, timeout=3600, dump_var=False, restore_var=None,
+        **params) -> dict:
+    """Run the reference driver; returns a dict of numpy arrays.
+
+    dump_var: also return the bytes of data/var.dat written after the run with
+    data_dump's statements ("var_dat").  restore_var: bytes of a var.dat to
+    restart from with data_restore's statements (fil is then ignored; the run
+    continues at the restored itime + 1).  With tree_print=True and shots
+    dividing a step, the reference's data/tree_mesh%03d.log files come back
+    as {"tree_mesh": {number: text}}."""
     if not available():
         raise FileNotFoundError(DRIVER)
-    flags = (1 if dump_tree else 0) | (2 if timing else 0)
+    flags = ((1 if dump_tree else 0) | (2 if timing else 0) | (64 if dump_var else 0)
+             | (128 if restore_var is not None else 0))
     env = dict(os.environ)
     if threads is not None:
         env["OMP_NUM_THREADS"] = str(threads)
@@ -61,22 +71,37 @@ def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None
         with open(os.path.join(wd, "run.in"), "w") as fh:
             fh.write(run_in_text(nsteps=nsteps, **params))
         os.makedirs(os.path.join(wd, "data"), exist_ok=True)
+        if restore_var is not None:
+            with open(os.path.join(wd, "data", "var.dat"), "wb") as fh:
+                fh.write(restore_var)
+            from types import SimpleNamespace
+            fil = SimpleNamespace(pcount=0, x=np.zeros((0, 3)), infront=np.zeros(0, np.int32),
+                                  behind=np.zeros(0, np.int32))
         _write_state(os.path.join(wd, "state.bin"), fil, nsteps, flags, nsample, u1, u2, u3)
         proc = subprocess.run([DRIVER], cwd=wd, env=env, capture_output=True, text=True,
                               timeout=timeout)
         if proc.returncode != 0 or "FATAL" in proc.stdout:
             raise RuntimeError(f"reference driver failed: {proc.stdout[-2000:]}{proc.stderr[-2000:]}")
         out = {"stdout": proc.stdout}
+        meshes = {}
+        for name in sorted(os.listdir(os.path.join(wd, "data"))):
+            if name.startswith("tree_mesh") and name.endswith(".log"):
+                with open(os.path.join(wd, "data", name)) as fh:
+                    meshes[int(name[9:-4])] = fh.read()
+        if meshes:
+            out["tree_mesh"] = meshes
+        if dump_var:
+            with open(os.path.join(wd, "data", "var.dat"), "rb") as fh:
+                out["var_dat"] = fh.read()
         if timing:
             with open(os.path.join(wd, "timing.txt")) as fh:
                 for line in fh:
                     k, v = line.split()
                     out[k] = float(v)
             return out
-        n = fil.pcount
         raw = open(os.path.join(wd, "out.bin"), "rb").read()
-        pc = np.frombuffer(raw, dtype="<i4", count=1)[0]
-        assert pc == n
+        n = int(np.frombuffer(raw, dtype="<i4", count=1)[0])
+        assert restore_var is not None or n == fil.pcount
         out["evals"] = int(np.frombuffer(raw, dtype="<i8", count=1, offset=4)[0])
         arr = np.frombuffer(raw, dtype="<f8", offset=12).reshape(9, n, 3)
         for k, name in enumerate(["u", "u_li", "u_bs", "ghosti", "ghostb", "x", "u1", "u2", "u3"]):

@@ -6,13 +6,26 @@
 !> run.f90:36-169 for the hot path: velocity (ghostp + construct_tree +
 !> calc_velocity), the AB ladder of pmotion (or pmotion_RK3 with
 !> runge_kutta T, run.f90:50-54), the periodic wrap, t = t + dt,
-!> and every `shots` steps the ts.log line and the var%04d.log snapshot
-!> (output.f90:55-143).  Line operations (pinsert, precon, premove) are not
-!> part of this driver.
+!> and every `shots` steps the restart file data/var.dat (data_dump,
+!> output.f90:272-285), the ts.log line and the var%04d.log snapshot
+!> (output.f90:55-143).  With tree_print T, the leaves of the step's tree go
+!> to data/tree_mesh%03d.log as construct_tree prints them (tree.f90:116-123).
+!> A data/var.dat present at start-up restarts the run from it as
+!> init_setup does (initial.f90:168-169, data_restore initial.f90:500-529):
+!> the loop continues at the dumped itime + 1 (at 1 with restart_rewind T).
+!> Line operations (pinsert, precon, premove) are not part of this driver.
 !>
 !> run.in labels used: nsteps shots dt delta box_size velocity boundary
 !> quant_circ corea tree_theta tree_extra_correction deriv_order runge_kutta initf pcount
-!> and gpu_precision (fp64 | fp32c, this driver's own).
+!> tree_print restart_rewind and gpu_precision (fp64 | fp32c, this driver's own).
+!>
+!> var.dat holds f(:) as the reference's type qvort (cdata.f90:20-38); the
+!> components this driver does not carry (x_interp, acc, u_mf, u_s_LI,
+!> u_s_BS, the ghosts, f_mf, stretch, curv, closest*, pinned*, delta,
+!> t_recon) are written as zero, u_sup as u; a restart reads x, u, u1..u3,
+!> infront and behind.  With runge_kutta T the reference prints the leaves of
+!> all three stage trees into one tree_mesh file; this driver prints the first
+!> stage's tree (the one of the step's start).
 !> initf: single_loop (initial_loop.f90:11-39) or state_file (state.bin:
 !> int32 version, pcount, nsteps, flags, nsample; real64 x(3,pcount);
 !> int32 infront(pcount), behind(pcount); real64 u1, u2, u3 (3,pcount)).
@@ -33,6 +46,29 @@ program qv_run
   logical :: tree_extra_correction = .true., runge_kutta = .false.
   character(len=30) :: velocity = 'Tree', boundary = 'periodic', deriv_order = 'second'
   character(len=30) :: initf = 'single_loop', gpu_precision = 'fp64'
+  logical :: tree_print = .false., restart_rewind = .false.
+  ! type qvort (cdata.f90:20-38), component for component: the record layout
+  ! of f(:) in data/var.dat
+  type :: qvort_record
+    real(c_double) :: x(3), x_interp(3)
+    real(c_double) :: u(3), u1(3), u2(3), u3(3)
+    real(c_double) :: acc(3)
+    real(c_double) :: u_sup(3), u_mf(3)
+    real(c_double) :: u_s_LI(3), u_s_BS(3)
+    real(c_double) :: ghosti(3), ghostb(3)
+    real(c_double) :: ghostii(3), ghostbb(3)
+    real(c_double) :: f_mf(3), stretch, curv
+    integer(c_int32_t) :: infront, behind
+    integer(c_int32_t) :: closest
+    real(c_double) :: closestd
+    real(c_double) :: closestd_loop
+    logical(4) :: pinnedi
+    logical(4) :: pinnedb
+    real(c_double) :: delta
+    real(c_double) :: t_recon(2)
+  end type
+  integer(c_int32_t) :: recon_count = 0
+  integer :: nstart = 1
   ! state
   type(qv_params) :: p
   type(c_ptr) :: ctx
@@ -84,12 +120,20 @@ program qv_run
   write(*,'(a,i0,a,a,a,a)') ' qv_run: ', count(infront > 0), ' points, velocity ', &
         trim(velocity), ', precision ', trim(gpu_precision)
 
-  do itime = 1, nsteps
+  do itime = nstart, nsteps
     if (runge_kutta) then
+      if (tree_print .and. tree_theta > 0._c_double .and. mod(itime, shots) == 0) then
+        ! the step's first tree: the velocity stage at the step's start
+        call qv_check(qv_velocity(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(evals)), &
+                      'qv_velocity')
+        call print_tree_mesh(itime / shots)
+      end if
       call qv_check(qv_rk3_step(ctx), 'qv_rk3_step')
     else
       call qv_check(qv_velocity(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(evals)), &
                     'qv_velocity')
+      if (tree_print .and. tree_theta > 0._c_double .and. mod(itime, shots) == 0) &
+        call print_tree_mesh(itime / shots)
       call qv_check(qv_ab_step(ctx), 'qv_ab_step')
     end if
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
@@ -97,6 +141,7 @@ program qv_run
     if (mod(itime, shots) == 0) then
       call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
                     'qv_download')
+      call data_dump()
       call print_info(itime / shots)
       call printf(itime / shots)
     end if
@@ -161,6 +206,10 @@ contains
           read(buffer, *, iostat=ios) gpu_precision
         case ('runge_kutta')
           read(buffer, *, iostat=ios) runge_kutta
+        case ('tree_print')
+          read(buffer, *, iostat=ios) tree_print
+        case ('restart_rewind')
+          read(buffer, *, iostat=ios) restart_rewind
       end select
       ios = 0
     end do
@@ -172,6 +221,12 @@ contains
 
   subroutine init_filament()
     integer :: fh, version, nst, flags, nsample
+    logical :: restart
+    inquire(file='./data/var.dat', exist=restart)
+    if (restart) then
+      call data_restore()
+      return
+    end if
     select case (initf)
       case ('single_loop')
         ! setup_single_loop, initial_loop.f90:11-39
@@ -205,6 +260,111 @@ contains
       case default
         call stop_with('init_setup', 'initf must be single_loop or state_file')
     end select
+  end subroutine
+
+  !> data_restore's reads (initial.f90:506-525); quasi particles and
+  !> particles are not part of this driver and must be absent
+  subroutine data_restore()
+    integer :: fh, dummy_itime, quasi_pcount, part_count
+    type(qvort_record), allocatable :: rec(:)
+    open(newunit=fh, file='./data/var.dat', form='unformatted', status='old', action='read')
+    read(fh) pcount
+    read(fh) recon_count
+    read(fh) dummy_itime
+    read(fh) t
+    allocate(rec(pcount))
+    read(fh) rec
+    read(fh) quasi_pcount
+    if (quasi_pcount > 0) call stop_with('data_restore', 'quasi particles are not supported')
+    read(fh) part_count
+    if (part_count > 0) call stop_with('data_restore', 'particles are not supported')
+    close(fh)
+    call alloc_f()
+    do i = 1, pcount
+      x(:, i) = rec(i)%x
+      u(:, i) = rec(i)%u
+      u1(:, i) = rec(i)%u1
+      u2(:, i) = rec(i)%u2
+      u3(:, i) = rec(i)%u3
+      infront(i) = rec(i)%infront
+      behind(i) = rec(i)%behind
+    end do
+    have_hist = any(u1 /= 0._c_double) .or. any(u2 /= 0._c_double) .or. &
+                any(u3 /= 0._c_double)
+    if (restart_rewind) then
+      nstart = 1
+    else
+      nstart = dummy_itime + 1
+    end if
+    write(*,*) 'data read in from dump file at t=', t
+  end subroutine
+
+  !> data_dump's writes (output.f90:274-283)
+  subroutine data_dump()
+    integer :: fh
+    type(qvort_record), allocatable :: rec(:)
+    allocate(rec(pcount))
+    do i = 1, pcount
+      rec(i)%x = x(:, i)
+      rec(i)%x_interp = 0._c_double
+      rec(i)%u = u(:, i)
+      rec(i)%u1 = u1(:, i)
+      rec(i)%u2 = u2(:, i)
+      rec(i)%u3 = u3(:, i)
+      rec(i)%acc = 0._c_double
+      rec(i)%u_sup = u(:, i)
+      rec(i)%u_mf = 0._c_double
+      rec(i)%u_s_LI = 0._c_double
+      rec(i)%u_s_BS = 0._c_double
+      rec(i)%ghosti = 0._c_double
+      rec(i)%ghostb = 0._c_double
+      rec(i)%ghostii = 0._c_double
+      rec(i)%ghostbb = 0._c_double
+      rec(i)%f_mf = 0._c_double
+      rec(i)%stretch = 0._c_double
+      rec(i)%curv = 0._c_double
+      rec(i)%infront = infront(i)
+      rec(i)%behind = behind(i)
+      rec(i)%closest = 0
+      rec(i)%closestd = 0._c_double
+      rec(i)%closestd_loop = 0._c_double
+      rec(i)%pinnedi = .false.
+      rec(i)%pinnedb = .false.
+      rec(i)%delta = 0._c_double
+      rec(i)%t_recon = 0._c_double
+    end do
+    open(newunit=fh, file='./data/var.dat', form='unformatted', status='replace')
+    write(fh) pcount
+    write(fh) recon_count
+    write(fh) itime
+    write(fh) t
+    write(fh) rec
+    write(fh) 0_c_int32_t
+    write(fh) 0_c_int32_t
+    close(fh)
+  end subroutine
+
+  !> construct_tree's leaf print (tree.f90:116-123): every cell that is not
+  !> split, in the order the reference creates them (DFS), appended to
+  !> data/tree_mesh%03d.log as list-directed lines
+  subroutine print_tree_mesh(filenumber)
+    integer, intent(in) :: filenumber
+    character(len=40) :: print_file
+    integer :: fh
+    integer(c_int64_t) :: nn, k
+    real(c_double), allocatable, target :: nodes(:,:)
+    call qv_check(qv_tree_dump(ctx, c_null_ptr, 0_c_int64_t, nn), 'qv_tree_dump')
+    allocate(nodes(22, nn))
+    call qv_check(qv_tree_dump(ctx, c_loc(nodes), nn, nn), 'qv_tree_dump')
+    write(print_file, '(a,i3.3,a)') './data/tree_mesh', filenumber, '.log'
+    open(newunit=fh, file=print_file, action='write', position='append')
+    do k = 1, nn
+      if (nodes(2, k) > 1._c_double) cycle
+      write(fh, *) nodes(4, k), nodes(4, k) + nodes(7, k), nodes(5, k), &
+                   nodes(5, k) + nodes(7, k), nodes(6, k), nodes(6, k) + nodes(7, k), &
+                   int(nodes(2, k))
+    end do
+    close(fh)
   end subroutine
 
   subroutine alloc_f()

@@ -26,6 +26,11 @@ def closest_names():
         return sorted(json.load(fh))
 
 
+def fmt_names():
+    with open(os.path.join(GOLDEN, "fmt_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
 def probe_names():
     with open(os.path.join(GOLDEN, "probe_index.json")) as fh:
         return sorted(json.load(fh))

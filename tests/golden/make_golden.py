@@ -9,6 +9,7 @@ case records the run.in parameters it used.  Run from the repository root:
     python tests/golden/make_golden.py --closest    # neighbour-search cases
     python tests/golden/make_golden.py --probe      # velocity at probe points
     python tests/golden/make_golden.py --rk3        # pmotion_RK3 trajectories
+    python tests/golden/make_golden.py --formats    # var.dat and tree_mesh files
 
 The neighbour-search cases (closest_*.npz, index in closest_index.json) run
 one step of ghostp, construct_tree, pmotion, optionally pinsert, then
@@ -25,6 +26,14 @@ The Runge-Kutta cases (rk3_*.npz, index in rk3_index.json) run the main loop
 with runge_kutta T (pmotion_RK3, then the wrap) and store the input, the
 positions and the Runge-Kutta storage u1 after nsteps, and the velocity of
 the first step's last stage.
+
+The format cases (fmt_*.npz, index in fmt_index.json) run nsteps of the main
+loop with tree_print T and shots 1, then write data/var.dat with data_dump's
+statements, and restart from that file for one more step (data_restore's
+statements).  They store the input state, the var.dat bytes after nsteps,
+the positions and AB history after nsteps, the text of every
+data/tree_mesh%03d.log (mesh_<n>), and from the restarted run the positions
+after the extra step and its var.dat (var_dat_restart).
 """
 from __future__ import annotations
 
@@ -178,7 +187,47 @@ def main_rk3():
         json.dump(index, fh, indent=1, sort_keys=True)
 
 
+def fmt_cases():
+    holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=3, seed=71), 20, seed=7)
+    loops = G.random_loops(0.1, 5e-4, n_loops=2, seed=72)
+    base = dict(delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4, box_size=0.1)
+    return [
+        ("fmt_holes_tree_periodic", holes, dict(base, velocity="Tree", tree_theta=0.4,
+                                                boundary="periodic"), 2),
+        ("fmt_loops_bs_openx", loops, dict(base, velocity="BS", tree_theta=0.5,
+                                           boundary="openx"), 2),
+    ]
+
+
+def main_formats():
+    index = {}
+    for name, fil, params, nsteps in fmt_cases():
+        r = refrun.run(fil, nsteps=nsteps, threads=1, dump_var=True, tree_print=True, shots=1,
+                       **params)
+        rr = refrun.run(None, nsteps=1, threads=1, dump_var=True, tree_print=True, shots=1,
+                        restore_var=r["var_dat"], **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind,
+                      var_dat=np.frombuffer(r["var_dat"], dtype=np.uint8),
+                      x_final=r["x"], u1_final=r["u1"], u2_final=r["u2"], u3_final=r["u3"],
+                      x_restart=rr["x"],
+                      var_dat_restart=np.frombuffer(rr["var_dat"], dtype=np.uint8))
+        meshes = dict(r.get("tree_mesh", {}))
+        meshes.update(rr.get("tree_mesh", {}))
+        for k, text in meshes.items():
+            arrays[f"mesh_{k}"] = np.frombuffer(text.encode(), dtype=np.uint8)
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount, meshes=sorted(meshes))
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "meshes", sorted(meshes), flush=True)
+    with open(os.path.join(OUT, "fmt_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def main():
+    if "--formats" in sys.argv[1:]:
+        main_formats()
+        return
     if "--rk3" in sys.argv[1:]:
         main_rk3()
         return

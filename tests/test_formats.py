@@ -1,0 +1,129 @@
+"""Qvort's on-disk formats (qvort_amd.formats) against files the reference
+build wrote.
+
+The fixtures (tests/golden/fmt_*.npz) hold data/var.dat as the reference's
+type qvort and data_dump's statements wrote it after a few steps, the
+positions and AB history of the same run (written separately, element by
+element), and every data/tree_mesh%03d.log construct_tree printed.  The
+layout is pinned by finding those independent arrays at their offsets, the
+writer by byte equality and by a restart of the reference build from a
+file this module wrote, the leaf print by the oracle's own tree.
+"""
+import numpy as np
+import pytest
+
+from conftest import Fil, fmt_names, load_golden, oracle_params
+
+NAMES = fmt_names()
+
+
+def _t_after(nsteps, dt):
+    t = 0.0
+    for _ in range(nsteps):
+        t = t + dt
+    return t
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_var_dat_layout(name, tmp_path):
+    from qvort_amd import formats
+    g = load_golden(name)
+    meta = g["meta"]
+    path = tmp_path / "var.dat"
+    path.write_bytes(g["var_dat"].tobytes())
+    d = formats.read_var_dat(path)
+    assert d["pcount"] == meta["pcount"]
+    assert d["itime"] == meta["nsteps"]
+    assert d["t"] == _t_after(meta["nsteps"], meta["params"]["dt"])
+    assert d["recon_count"] == 0 and d["quasi_pcount"] == 0 and d["part_count"] == 0
+    f = d["f"]
+    assert np.array_equal(f["x"], g["x_final"])
+    assert np.array_equal(f["u1"], g["u1_final"])
+    assert np.array_equal(f["u2"], g["u2_final"])
+    assert np.array_equal(f["u3"], g["u3_final"])
+    assert np.array_equal(f["infront"], g["infront"])
+    assert np.array_equal(f["behind"], g["behind"])
+    # pmotion shifts u into u1 (timestep.f90:68-70): u sits where u1 does
+    assert np.array_equal(f["u"], f["u1"])
+    assert not f["pinnedi"].any() and not f["pinnedb"].any()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_var_dat_writer_reproduces_bytes(name, tmp_path):
+    from qvort_amd import formats
+    g = load_golden(name)
+    src = tmp_path / "in.dat"
+    src.write_bytes(g["var_dat"].tobytes())
+    d = formats.read_var_dat(src)
+    out = tmp_path / "out.dat"
+    formats.write_var_dat(out, d["f"], itime=d["itime"], t=d["t"], recon_count=d["recon_count"])
+    assert out.read_bytes() == g["var_dat"].tobytes()
+
+
+def test_var_dat_rejects_damaged_files(tmp_path):
+    from qvort_amd import formats
+    g = load_golden(NAMES[0])
+    raw = g["var_dat"].tobytes()
+    bad = tmp_path / "bad.dat"
+    bad.write_bytes(raw[:-3])
+    with pytest.raises(formats.FormatError):
+        formats.read_var_dat(bad)
+    mixed = bytearray(raw)
+    mixed[-1] ^= 1  # the closing marker of the last record
+    bad.write_bytes(bytes(mixed))
+    with pytest.raises(formats.FormatError):
+        formats.read_var_dat(bad)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_reference_restarts_from_written_file(name, tmp_path):
+    """A var.dat built by records_from_state from the fixture's state: the
+    reference build restores it with data_restore's reads and its next step
+    is the one of the fixture's own restarted run, bit for bit."""
+    from oracle import refrun
+    from qvort_amd import formats
+    if not refrun.available():
+        pytest.skip("reference build not present")
+    g = load_golden(name)
+    meta = g["meta"]
+    path = tmp_path / "var.dat"
+    f = formats.records_from_state(g["x_final"], g["infront"], g["behind"], u=g["u1_final"],
+                                   u1=g["u1_final"], u2=g["u2_final"], u3=g["u3_final"])
+    formats.write_var_dat(path, f, itime=meta["nsteps"],
+                          t=_t_after(meta["nsteps"], meta["params"]["dt"]))
+    r = refrun.run(None, nsteps=1, threads=1, restore_var=path.read_bytes(), **meta["params"])
+    assert np.array_equal(r["x"], g["x_restart"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_tree_mesh_rows_match_reference_print(name, oracle_lib):
+    """The leaf print of the first step (tree of the input) and of the
+    restarted step (tree of the positions after nsteps) against the oracle's
+    node dump of the same positions."""
+    from oracle import port
+    from qvort_amd import formats
+    g = load_golden(name)
+    meta = g["meta"]
+    p = oracle_params(dict(meta["params"], velocity="Tree"))
+    for k, x in ((1, g["x0"]), (meta["nsteps"] + 1, g["x_final"])):
+        text = g[f"mesh_{k}"].tobytes().decode()
+        ref = formats.read_tree_mesh(text)
+        rows = formats.tree_mesh_rows(port.tree_dump(p, Fil(x, g["infront"], g["behind"])))
+        assert rows.shape == ref.shape
+        assert np.array_equal(rows, ref)
+
+
+def test_var_log_round_trip(tmp_path):
+    from qvort_amd import formats
+    rng = np.random.Generator(np.random.PCG64(73))
+    n = 37
+    x = rng.normal(size=(n, 3))
+    u = rng.normal(size=(n, 3))
+    inf = rng.integers(0, n, size=n).astype(np.int32)
+    path = tmp_path / "var0001.log"
+    formats.write_var_log(path, 1.5e-6, x, inf, u)
+    d = formats.read_var_log(path)
+    assert d["t"] == 1.5e-6 and d["pcount"] == n
+    assert np.array_equal(d["x"], x) and np.array_equal(d["u"], u)
+    assert np.array_equal(d["infront"], inf)
+    assert path.stat().st_size == 12 + n * (14 * 8 + 4)
