@@ -63,7 +63,10 @@ enum qv_status {
                             would need more than 32 levels */
   QV_ERR_STATE = 5,      /* call out of order (e.g. qv_ab_step before any
                             qv_velocity) */
-  QV_ERR_NO_DEVICE = 6   /* no HIP device visible */
+  QV_ERR_NO_DEVICE = 6,  /* no HIP device visible */
+  QV_ERR_LINE_EMPTY = 7, /* no active point: the fatal_error of line.f90:27
+                            ("vortex line length is 0, run over") */
+  QV_ERR_CAPACITY = 8    /* pinsert needs more slots than the arrays hold */
 };
 
 /* the run.in parameters of the hot path (cdata.f90:196-416) */
@@ -230,6 +233,54 @@ int qv_batch_set_variant(qv_batch *batch, int32_t variant);
  * work units per launch, device time of the last qv_batch_step call */
 int qv_batch_info(qv_batch *batch, int32_t *nvariants, int32_t *variant, int32_t *tile,
                   int32_t *nunits, float *last_ms);
+
+/* ---- line operations of a step (run.f90:60-72), host code on host arrays.
+ * The reference's topology step: pinsert, then every recon_shots steps the
+ * neighbour search (qv_closest on the device tree), precon and premove.
+ * They are branchy list surgery on a few points per step and run on the
+ * host as in the reference; they read the ghost zones the step's ghostp
+ * left (qv_download_ghosts), which pinsert refreshes only for the points it
+ * touches, exactly as the reference does.  Run.in options they implement:
+ * recon_type 'original', delta_adapt F, phonon_emission F, recon_info F,
+ * no mirror; the diagnostic logs (removed_curv.log, small_loops.log) are
+ * not written. */
+typedef struct qv_line_state {
+  int32_t pcount;    /* slots in use (size(f)); pinsert may raise it */
+  int32_t capacity;  /* slots allocated in every array below */
+  double *x, *u, *u1, *u2, *u3;                /* (3, capacity) each */
+  double *ghosti, *ghostb, *ghostii, *ghostbb; /* (3, capacity) each */
+  int32_t *infront, *behind, *closest;         /* (capacity), 1-based */
+  double *closestd;                            /* (capacity) */
+} qv_line_state;
+
+/* the reference's module counters (cdata.f90:146-154, 314-315); calls add
+ * to the counts, pinsert overwrites total_length and avg_sep */
+typedef struct qv_line_counts {
+  int64_t inserted;                /* points added by pinsert */
+  int64_t recon_count;             /* reconnections (recon_count) */
+  int64_t remove_count;            /* points removed by premove (remove_count) */
+  int64_t small_loop_remove_count; /* loops of 3-4 points removed by loop_killer */
+  double total_length;             /* sum of segment lengths seen by pinsert */
+  double avg_sep;                  /* total_length / active points (pinsert) */
+  double small_loop_remove_length; /* length of the loops loop_killer removed */
+} qv_line_counts;
+
+/* ghost zones of the last ghostp (velocity stage or QV_PROBE_GHOSTP), in
+ * the layout of x(3,pcount); NULL skips an array.  QV_ERR_STATE if none. */
+int qv_download_ghosts(qv_ctx *ctx, double *gi, double *gb, double *gii, double *gbb);
+/* pinsert (line.f90:18-108): a point between i and infront(i) where their
+ * separation exceeds delta, curvature-placed, into the first empty slot or
+ * appended.  Each slot adds at most one point, so capacity >= 2 pcount
+ * always suffices; with less, QV_ERR_CAPACITY is returned when a point does
+ * not fit and the points inserted before it stay (pcount counts them). */
+int qv_line_pinsert(const qv_params *params, qv_line_state *s, qv_line_counts *c);
+/* precon_original (reconnection.f90:145-256) with loop_killer
+ * (reconnection.f90:421-461): reconnects i with closest(i) when closestd(i)
+ * < delta/2, the tangents are not parallel and the line does not grow */
+int qv_line_precon(const qv_params *params, qv_line_state *s, qv_line_counts *c);
+/* premove (line.f90:114-161) with loop_killer: removes infront(i) when i and
+ * infront(infront(i)) are closer than 0.998 delta */
+int qv_line_premove(const qv_params *params, qv_line_state *s, qv_line_counts *c);
 
 int qv_get_timings(qv_ctx *ctx, qv_timings *t);
 const char *qv_status_string(int status);

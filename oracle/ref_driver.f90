@@ -26,6 +26,13 @@
 !>                   own type(qvort) under this compiler.
 !>   construct_tree's leaf print data/tree_mesh%03d.log (tree.f90:116-123)
 !>                   when run.in sets tree_print T and shots divides itime
+!>   the line operations of run.f90:60-72 (flags bit 8): pinsert
+!>                   (line.f90:18-108), and when recon_shots divides itime
+!>                   pclose_tree / pclose, precon (line.f90:238-252, run.in
+!>                   recon_type, default precon_original reconnection.f90:145-256)
+!>                   and premove (line.f90:114-161, loop_killer
+!>                   reconnection.f90:421-461), between pmotion and the final
+!>                   ghostp + wrap of every step
 !> and mirrors the order of the main loop in run.f90:36-169 with the line
 !> operations left out, except in the neighbour-search mode (flags bit 2):
 !> one step of ghostp, construct_tree, pmotion, [pinsert], then the search,
@@ -45,6 +52,16 @@
 !>              real64 x(3,pcount); int32 infront, behind, closest (before the
 !>              search); real64 closestd (before); int32 closest (after);
 !>              real64 closestd (after)
+!>   line.bin   (flags bit 8) the state after pmotion, pinsert, the search,
+!>              precon and premove of the first step (each: int32 pcount;
+!>              real64 x, u, u1, u2, ghosti, ghostb, ghostii, ghostbb
+!>              (3,pcount each); int32 infront, behind, closest; real64
+!>              closestd), the counters after the first step, then after
+!>              nsteps: int32 pcount; real64 x, u1, u2 (3,pcount);
+!>              int32 infront, behind; counters.  Counters: int32
+!>              recon_count, remove_count, small_loop_remove_count; real64
+!>              total_length, avg_sep, small_loop_remove_length.  out.bin is
+!>              not written in this mode.
 !>   probes.bin (flags bit 4, read) int32 nprobe; real64 y(3,nprobe)
 !>   probe.bin  (flags bit 4) int32 pcount; real64 x(3,pcount), ghosti(3,pcount)
 !>              as the probe sums saw them; real64 u(3,nprobe).  The sums
@@ -161,6 +178,27 @@ program qv_ref_driver
       end do
       if (tree_theta > 0) eval_first = eval_counter
     end if
+    if (btest(flags, 8)) then
+      if (itime == nstart) then
+        open(unit=48, file='line.bin', access='stream', form='unformatted', status='replace')
+        call dump_line_state()
+      end if
+      call pinsert
+      if (itime == nstart) call dump_line_state()
+      if (mod(itime, recon_shots) == 0) then
+        if (tree_theta > 0) then
+          call pclose_tree
+        else
+          call pclose
+        end if
+        if (itime == nstart) call dump_line_state()
+        call precon
+        if (itime == nstart) call dump_line_state()
+        call premove
+        if (itime == nstart) call dump_line_state()
+      end if
+      if (itime == nstart) call dump_line_counters()
+    end if
     call ghostp
     select case(boundary)
       case('periodic')
@@ -194,6 +232,24 @@ program qv_ref_driver
     close(53)
   end if
 
+  if (btest(flags, 8)) then
+    write(48) pcount
+    do i = 1, pcount
+      write(48) f(i)%x
+    end do
+    do i = 1, pcount
+      write(48) f(i)%u1
+    end do
+    do i = 1, pcount
+      write(48) f(i)%u2
+    end do
+    write(48) (f(i)%infront, i = 1, pcount)
+    write(48) (f(i)%behind, i = 1, pcount)
+    call dump_line_counters()
+    close(48)
+    stop
+  end if
+
   open(unit=43, file='out.bin', access='stream', form='unformatted', status='replace')
   write(43) pcount
   write(43) eval_first
@@ -224,6 +280,45 @@ contains
       case('openxy')
         periodic_bc_notxy = .true.
     end select
+  end subroutine
+
+  !> the f(:) fields the line operations read or write, to line.bin
+  subroutine dump_line_state()
+    integer :: q
+    write(48) pcount
+    do q = 1, pcount
+      write(48) f(q)%x
+    end do
+    do q = 1, pcount
+      write(48) f(q)%u
+    end do
+    do q = 1, pcount
+      write(48) f(q)%u1
+    end do
+    do q = 1, pcount
+      write(48) f(q)%u2
+    end do
+    do q = 1, pcount
+      write(48) f(q)%ghosti
+    end do
+    do q = 1, pcount
+      write(48) f(q)%ghostb
+    end do
+    do q = 1, pcount
+      write(48) f(q)%ghostii
+    end do
+    do q = 1, pcount
+      write(48) f(q)%ghostbb
+    end do
+    write(48) (f(q)%infront, q = 1, pcount)
+    write(48) (f(q)%behind, q = 1, pcount)
+    write(48) (f(q)%closest, q = 1, pcount)
+    write(48) (f(q)%closestd, q = 1, pcount)
+  end subroutine
+
+  subroutine dump_line_counters()
+    write(48) recon_count, remove_count, small_loop_remove_count
+    write(48) total_length, avg_sep, small_loop_remove_length
   end subroutine
 
   !> every node (empty children included) in the DFS order of tree_walk

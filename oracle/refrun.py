@@ -24,9 +24,10 @@ def available() -> bool:
 def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.1,
                 delta=5e-4, dt=1e-6, quant_circ=9.97e-4, corea=8.244023e-9,
                 tree_extra_correction=True, deriv_order="second", nsteps=1,
-                runge_kutta=False, shots=1000000, tree_print=False) -> str:
+                runge_kutta=False, shots=1000000, tree_print=False,
+                recon_shots=1000000) -> str:
     lines = [f"runge_kutta {'T' if runge_kutta else 'F'}",
-        f"nsteps {nsteps}", f"shots {shots}", "recon_shots 1000000", f"dt {dt!r}",
+        f"nsteps {nsteps}", f"shots {shots}", f"recon_shots {recon_shots}", f"dt {dt!r}",
         f"tree_print {'T' if tree_print else 'F'}",
         f"delta {delta!r}", f"box_size {box_size!r}", f"velocity {velocity}",
         f"boundary {boundary}", f"quant_circ {quant_circ!r}", f"corea {corea!r}",
@@ -197,4 +198,68 @@ def probe_run(fil, probes, *, motion=False, threads=1, timeout=600, **params) ->
            "ghosti": arr[3 * n:6 * n].reshape(n, 3).copy(),
            "u": arr[6 * n:].reshape(-1, 3).copy()}
     assert out["u"].shape == y.shape
+    return out
+
+
+LINE_STAGES = ("motion", "pinsert", "search", "precon", "premove")
+
+
+def line_run(fil, *, nsteps=1, threads=1, timeout=600, u1=None, u2=None, u3=None,
+             **params) -> dict:
+    """The main loop with the line operations of run.f90:60-72 (pinsert, then
+    when recon_shots divides itime the neighbour search, precon and premove).
+    Returns the state after each operation of the first step
+    ({stage: {pcount, x, u, u1, u2, ghosti, ghostb, ghostii, ghostbb,
+    infront, behind, closest, closestd}} for the stages in LINE_STAGES that
+    ran), the counters after the first step ("counters1") and the state and
+    counters after nsteps ("final")."""
+    if not available():
+        raise FileNotFoundError(DRIVER)
+    flags = 256
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory(prefix="qvref_") as wd:
+        with open(os.path.join(wd, "run.in"), "w") as fh:
+            fh.write(run_in_text(nsteps=nsteps, **params))
+        os.makedirs(os.path.join(wd, "data"), exist_ok=True)
+        _write_state(os.path.join(wd, "state.bin"), fil, nsteps, flags, 0, u1, u2, u3)
+        proc = subprocess.run([DRIVER], cwd=wd, env=env, capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0 or "FATAL" in proc.stdout:
+            raise RuntimeError(f"reference driver failed: {proc.stdout[-2000:]}{proc.stderr[-2000:]}")
+        raw = open(os.path.join(wd, "line.bin"), "rb").read()
+    off = 0
+
+    def take(dtype, count, shape=None):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        return a if shape is None else a.reshape(shape)
+
+    def counters():
+        c = take("<i4", 3)
+        r = take("<f8", 3)
+        return {"recon_count": int(c[0]), "remove_count": int(c[1]),
+                "small_loop_remove_count": int(c[2]), "total_length": float(r[0]),
+                "avg_sep": float(r[1]), "small_loop_remove_length": float(r[2])}
+
+    recon = params.get("recon_shots", 1000000)
+    stages = LINE_STAGES if recon <= 1 or 1 % recon == 0 else LINE_STAGES[:2]
+    out = {}
+    for st in stages:
+        n = int(take("<i4", 1)[0])
+        d = {"pcount": n}
+        for name in ("x", "u", "u1", "u2", "ghosti", "ghostb", "ghostii", "ghostbb"):
+            d[name] = take("<f8", 3 * n, (n, 3))
+        for name in ("infront", "behind", "closest"):
+            d[name] = take("<i4", n)
+        d["closestd"] = take("<f8", n)
+        out[st] = d
+    out["counters1"] = counters()
+    n = int(take("<i4", 1)[0])
+    fin = {"pcount": n, "x": take("<f8", 3 * n, (n, 3)), "u1": take("<f8", 3 * n, (n, 3)),
+           "u2": take("<f8", 3 * n, (n, 3)), "infront": take("<i4", n), "behind": take("<i4", n)}
+    fin.update(counters())
+    out["final"] = fin
+    assert off == len(raw)
     return out

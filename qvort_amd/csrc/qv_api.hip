@@ -447,6 +447,22 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, doubl
   return QV_OK;
 }
 
+int qv_download_ghosts(qv_ctx *ctx, double *gi, double *gb, double *gii, double *gbb) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0 || !c->ghosts_valid) return QV_ERR_STATE;
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  double *hs[] = {gi, gb, gii, gbb};
+  Vec3 vs[] = {c->gi, c->gb, c->gii, c->gbb};
+  for (int k = 0; k < 4; ++k) {
+    if (!hs[k]) continue;
+    int rc = download_vec(*c, vs[k], hs[k]);
+    if (rc) return rc;
+    TRY(hipStreamSynchronize(c->stream));
+  }
+  return QV_OK;
+}
+
 int qv_rk3_step(qv_ctx *ctx) {
   if (!ctx) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -612,6 +628,8 @@ const char *qv_status_string(int status) {
     case QV_ERR_TREE_DEPTH: return "points closer than box_size/2^32: octree deeper than 32 levels";
     case QV_ERR_STATE: return "call out of order";
     case QV_ERR_NO_DEVICE: return "no HIP device";
+    case QV_ERR_LINE_EMPTY: return "line.mod:pinsert: vortex line length is 0, run over";
+    case QV_ERR_CAPACITY: return "line operations need more slots than the arrays hold";
     default: return "unknown status";
   }
 }

@@ -37,22 +37,23 @@ __device__ inline Dual log(Dual a) { return Dual(::log(a.v), a.d / a.v); }
 }  // namespace ad
 using ad::Dual;
 
+// The templates below are host and device code: the double instantiation
+// also serves the host line operations (pinsert's curvature, precon's
+// tangents), which must form s' and s'' exactly as the velocity does.
 template <class T>
-__device__ inline T lia_sq(T a) { return a * a; }
+__host__ __device__ inline T lia_sq(T a) { return a * a; }
 
 // dist_gen, derivatives.f90:79-86
 template <class T>
-__device__ inline T lia_dist(const T *a, const T *b) {
+__host__ __device__ inline T lia_dist(const T *a, const T *b) {
   return sqrt((lia_sq(a[0] - b[0]) + lia_sq(a[1] - b[1])) + lia_sq(a[2] - b[2]));
 }
 
-// get_deriv_1 / get_deriv_2 (derivatives.f90:11-75) and
-// u_LI = Gamma/(4 pi) ln((1.21/a) sqrt(l_i l_b)) s' x s''
+// get_deriv_1 (s1) and get_deriv_2 (s2), derivatives.f90:11-75
 template <class T>
-__device__ void lia_term(double qc, double corea, int order, const T x[3], const T gi[3],
-                         const T gb[3], const T gii[3], const T gbb[3], T u[3]) {
+__host__ __device__ void lia_derivs(int order, const T x[3], const T gi[3], const T gb[3],
+                                    const T gii[3], const T gbb[3], T s1[3], T s2[3]) {
   const T di = lia_dist(x, gi), db = lia_dist(x, gb);
-  T s1[3], s2[3];
   if (order != 4) {
     const T den = (T(2.) * db) * di;
     for (int k = 0; k < 3; ++k) s1[k] = ((db * gi[k] + (di - db) * x[k]) - di * gb[k]) / den;
@@ -89,6 +90,14 @@ __device__ void lia_term(double qc, double corea, int order, const T x[3], const
         s2[k] = (((A * gbb[k] + B * gb[k]) + C * x[k]) + D * gi[k]) + E * gii[k];
     }
   }
+}
+
+// u_LI = Gamma/(4 pi) ln((1.21/a) sqrt(l_i l_b)) s' x s''
+template <class T>
+__device__ void lia_term(double qc, double corea, int order, const T x[3], const T gi[3],
+                         const T gb[3], const T gii[3], const T gbb[3], T u[3]) {
+  T s1[3], s2[3];
+  lia_derivs<T>(order, x, gi, gb, gii, gbb, s1, s2);
   const T li = lia_dist(x, gi), lb = lia_dist(x, gb);
   const T beta = T(qc / (4. * 3.14159265358979324)) * log(T(1.21 / corea) * sqrt(li * lb));
   u[0] = beta * (s1[1] * s2[2] - s1[2] * s2[1]);

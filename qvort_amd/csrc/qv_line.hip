@@ -1,0 +1,277 @@
+// qv_line.hip -- the line operations of a step on host arrays: pinsert
+// (line.f90:18-108), precon_original (reconnection.f90:145-256), premove
+// (line.f90:114-161) and loop_killer (reconnection.f90:421-461).
+//
+// Host code.  These loops visit the slots in order and each change of the
+// links is seen by the slots visited after it, so the result depends on the
+// order (which slot a new point takes, which reconnection happens first);
+// they run sequentially, in the reference's order, and keep its FP64
+// operation order (no contraction), so that the state after each operation
+// is the reference's bit for bit.  Slot indices are 0-based here, links
+// 1-based as in f(:).
+#include <cmath>
+#include <cstring>
+
+#include "qv_lia.h"
+#include "qvort_gpu.h"
+
+#pragma clang fp contract(off)
+
+namespace qv {
+namespace {
+
+struct Line {
+  const qv_params &p;
+  qv_line_state &s;
+  qv_line_counts &c;
+  bool per[3];
+
+  Line(const qv_params &p_, qv_line_state &s_, qv_line_counts &c_) : p(p_), s(s_), c(c_) {
+    per[0] = p.boundary == QV_BC_PERIODIC;
+    per[1] = p.boundary == QV_BC_PERIODIC || p.boundary == QV_BC_OPENX;
+    per[2] = p.boundary != QV_BC_OPEN;
+  }
+  double *X(int i) { return s.x + 3 * (size_t)i; }
+  double *U(int i) { return s.u + 3 * (size_t)i; }
+  double *GI(int i) { return s.ghosti + 3 * (size_t)i; }
+  double *GB(int i) { return s.ghostb + 3 * (size_t)i; }
+  double *GII(int i) { return s.ghostii + 3 * (size_t)i; }
+  double *GBB(int i) { return s.ghostbb + 3 * (size_t)i; }
+  int inf(int i) const { return s.infront[i] - 1; }
+  int beh(int i) const { return s.behind[i] - 1; }
+
+  // get_ghost_p (periodic.f90:23-184): the neighbours' positions, each
+  // moved by one box length where it lies more than half a box away from
+  // x(i) along a periodic axis
+  void ghost_p(int i) {
+    const int a = inf(i), b = beh(i), aa = inf(a), bb = beh(b);
+    const double *x = X(i);
+    const int src[4] = {a, b, aa, bb};
+    double *dst[4] = {GI(i), GB(i), GII(i), GBB(i)};
+    const double L = p.box_size;
+    for (int q = 0; q < 4; ++q) {
+      const double *g = X(src[q]);
+      for (int k = 0; k < 3; ++k) {
+        double v = g[k];
+        if (per[k]) {
+          if ((x[k] - v) > (L / 2.)) v = v + L;
+          else if ((x[k] - v) < (-L / 2.)) v = v - L;
+        }
+        dst[q][k] = v;
+      }
+    }
+  }
+
+  void derivs(int i, double s1[3], double s2[3]) {
+    lia_derivs<double>(p.deriv_order, X(i), GI(i), GB(i), GII(i), GBB(i), s1, s2);
+  }
+
+  // distf (general.f90:27-34)
+  double distf(int i, int j) {
+    const double *a = X(i), *b = X(j);
+    return std::sqrt(((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1])) +
+                     (a[2] - b[2]) * (a[2] - b[2]));
+  }
+
+  // clear_particle (general.f90:11-22); u3 is left as it is
+  void clear(int i) {
+    for (double *v : {X(i), U(i), s.u1 + 3 * (size_t)i, s.u2 + 3 * (size_t)i, GI(i), GB(i),
+                      GII(i), GBB(i)})
+      v[0] = v[1] = v[2] = 0.;
+    s.infront[i] = s.behind[i] = 0;
+    s.closest[i] = 0;
+    s.closestd[i] = 0.;
+  }
+
+  // loop_killer (reconnection.f90:421-461): a loop of fewer than 5 points
+  // is removed.  On a slot that is already empty the reference follows the
+  // link 0 out of the array; here that call does nothing.
+  // A chain that reaches an empty slot (not a closed loop, which the
+  // operations never leave behind) is not followed past it.
+  void loop_killer(int particle) {
+    if (s.infront[particle] == 0) return;
+    int counter = 1, next = particle;
+    for (int it = 0; it < s.pcount; ++it) {
+      if (s.infront[next] == 0) return;
+      next = inf(next);
+      if (next == particle) break;
+      ++counter;
+    }
+    if (counter >= 5) return;
+    next = particle;
+    if (counter > 2) c.small_loop_remove_count += 1;
+    for (int it = 0; it < s.pcount; ++it) {
+      const int store_next = inf(next);
+      if (store_next < 0) {
+        clear(next);
+        break;
+      }
+      if (counter > 2 && store_next != particle)
+        c.small_loop_remove_length = c.small_loop_remove_length + distf(next, store_next);
+      clear(next);
+      next = store_next;
+      if (next == particle) break;
+    }
+  }
+
+  int pinsert() {
+    int nact = 0;
+    for (int i = 0; i < s.pcount; ++i) nact += s.infront[i] > 0;
+    if (nact == 0) return QV_ERR_LINE_EMPTY;
+    const int old = s.pcount;
+    double total = 0.;
+    for (int i = 0; i < old; ++i) {
+      if (s.infront[i] == 0) continue;
+      const double disti = lia_dist(X(i), GI(i));
+      total = total + disti;
+      // the point's delta factor is 1 (delta_adapt F)
+      if (!(disti > 1. * p.delta)) continue;
+      // the first empty slot (minloc of infront), else one more slot
+      int nw = -1;
+      for (int k = 0; k < s.pcount; ++k)
+        if (s.infront[k] == 0) { nw = k; break; }
+      if (nw < 0) {
+        if (s.pcount >= s.capacity) return QV_ERR_CAPACITY;
+        nw = s.pcount++;
+        s.infront[nw] = 0;
+        s.u3[3 * (size_t)nw] = s.u3[3 * (size_t)nw + 1] = s.u3[3 * (size_t)nw + 2] = 0.;
+        s.closest[nw] = 0;
+        s.closestd[nw] = 0.;
+      }
+      double s1[3], fdd[3];
+      derivs(i, s1, fdd);
+      double curv = std::sqrt((fdd[0] * fdd[0] + fdd[1] * fdd[1]) + fdd[2] * fdd[2]);
+      const double *xi = X(i), *gi = GI(i);
+      double *xn = X(nw);
+      bool linear = true;
+      if (curv > 1e-5) {
+        curv = 1. / curv;
+        if (curv * curv - 0.25 * (disti * disti) > 0.) {
+          const double f = (std::sqrt(curv * curv - 0.25 * (disti * disti)) - curv) * curv;
+          for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (xi[k] + gi[k]) + f * fdd[k];
+          linear = false;
+        }
+      }
+      if (linear)
+        for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (xi[k] + gi[k]);
+      const int a = inf(i);
+      for (int k = 0; k < 3; ++k) U(nw)[k] = 0.5 * (U(i)[k] + U(a)[k]);
+      for (int k = 0; k < 3; ++k) s.u1[3 * (size_t)nw + k] = s.u2[3 * (size_t)nw + k] = 0.;
+      s.behind[nw] = i + 1;
+      s.infront[nw] = s.infront[i];
+      ghost_p(nw);
+      s.behind[a] = nw + 1;
+      ghost_p(a);
+      s.infront[i] = nw + 1;
+      ghost_p(i);
+      c.inserted += 1;
+    }
+    int empty = 0;
+    for (int k = 0; k < s.pcount; ++k) empty += s.infront[k] == 0;
+    c.total_length = total;
+    c.avg_sep = total / (double)(old - empty);
+    return QV_OK;
+  }
+
+  // norm_tanf (general.f90:94-107)
+  void norm_tan(int i, double t[3]) {
+    double s2[3];
+    derivs(i, t, s2);
+    const double len = std::sqrt((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
+    for (int k = 0; k < 3; ++k) t[k] = t[k] / len;
+  }
+
+  int precon() {
+    const double eps = 2.220446049250313e-16;  // epsilon(1.) in FP64
+    for (int i = 0; i < s.pcount; ++i) {
+      if (s.infront[i] == 0) continue;
+      const int pari = inf(i), parb = beh(i);
+      if (!(s.closestd[i] < p.delta / 2. && s.closestd[i] > eps)) continue;
+      const int j1 = s.closest[i];  // 1-based, 0 = none
+      if (j1 == pari + 1 || j1 == parb + 1 || j1 == 0) continue;
+      const int j = j1 - 1;
+      if (s.infront[j] == 0) continue;
+      const int parjb = beh(j);
+      double t1[3], t2[3];
+      norm_tan(i, t1);
+      norm_tan(j, t2);
+      const double dot = (t1[0] * t2[0] + t1[1] * t2[1]) + t1[2] * t2[2];
+      if (dot > 0.9) continue;  // parallel filaments do not reconnect
+      const double lb = ((lia_dist(X(i), GI(i)) + lia_dist(X(i), GB(i))) +
+                         lia_dist(X(j), GI(j))) + lia_dist(X(j), GB(j));
+      const double la = ((lia_dist(X(i), GB(i)) + lia_dist(X(i), X(j))) +
+                         lia_dist(X(j), GI(j))) + lia_dist(X(pari), X(parjb));
+      if (!(la <= lb)) continue;
+      c.recon_count += 1;
+      s.infront[parjb] = pari + 1;
+      s.behind[pari] = parjb + 1;
+      s.infront[i] = j + 1;
+      s.behind[j] = i + 1;
+      loop_killer(pari);
+      loop_killer(i);
+    }
+    return QV_OK;
+  }
+
+  int premove() {
+    for (int i = 0; i < s.pcount; ++i) {
+      if (s.infront[i] == 0) continue;
+      const int a = inf(i), aa = inf(a);
+      const double distii = distf(i, aa);
+      // both points' delta factors are 1 (delta_adapt F)
+      if (!(distii < 0.499 * p.delta * (1. + 1.))) continue;
+      s.behind[aa] = i + 1;
+      s.infront[i] = aa + 1;
+      clear(a);
+      loop_killer(i);
+      c.remove_count += 1;
+    }
+    return QV_OK;
+  }
+};
+
+// the arrays the operations index must be present and the links must stay
+// inside the slots in use
+int check(const qv_params *p, qv_line_state *s, qv_line_counts *c) {
+  if (!p || !s || !c) return QV_ERR_ARG;
+  if (s->pcount < 0 || s->capacity < s->pcount) return QV_ERR_ARG;
+  if (!s->x || !s->u || !s->u1 || !s->u2 || !s->u3 || !s->ghosti || !s->ghostb ||
+      !s->ghostii || !s->ghostbb || !s->infront || !s->behind || !s->closest || !s->closestd)
+    return QV_ERR_ARG;
+  if (p->deriv_order != 2 && p->deriv_order != 4) return QV_ERR_ARG;
+  if (p->boundary < QV_BC_OPEN || p->boundary > QV_BC_OPENXY || !(p->box_size > 0.))
+    return QV_ERR_ARG;
+  for (int i = 0; i < s->pcount; ++i) {
+    const int a = s->infront[i], b = s->behind[i];
+    if (a < 0 || a > s->pcount || b < 0 || b > s->pcount) return QV_ERR_ARG;
+    if ((a == 0) != (b == 0)) return QV_ERR_ARG;
+    if (a != 0 && (s->infront[a - 1] == 0 || s->behind[b - 1] == 0)) return QV_ERR_ARG;
+    if (s->closest[i] < 0 || s->closest[i] > s->pcount) return QV_ERR_ARG;
+  }
+  return QV_OK;
+}
+
+}  // namespace
+}  // namespace qv
+
+extern "C" {
+
+int qv_line_pinsert(const qv_params *params, qv_line_state *s, qv_line_counts *c) {
+  const int rc = qv::check(params, s, c);
+  if (rc) return rc;
+  return qv::Line(*params, *s, *c).pinsert();
+}
+
+int qv_line_precon(const qv_params *params, qv_line_state *s, qv_line_counts *c) {
+  const int rc = qv::check(params, s, c);
+  if (rc) return rc;
+  return qv::Line(*params, *s, *c).precon();
+}
+
+int qv_line_premove(const qv_params *params, qv_line_state *s, qv_line_counts *c) {
+  const int rc = qv::check(params, s, c);
+  if (rc) return rc;
+  return qv::Line(*params, *s, *c).premove();
+}
+
+}  // extern "C"

@@ -8,7 +8,9 @@
 !> qv_ab_step the Adams-Bashforth ladder of pmotion (timestep.f90:48-71),
 !> qv_enforce_periodic the wrap of enforce_periodic (periodic.f90:188-213),
 !> qv_closest the neighbour search pclose_tree / pclose (tree.f90:219-278,
-!> line.f90:205-234) that precon reads, and qv_destroy the empty_tree /
+!> line.f90:205-234) that precon reads, qv_line_pinsert / qv_line_precon /
+!> qv_line_premove the host line operations of run.f90:60-72 (pinsert,
+!> precon_original, premove with loop_killer), and qv_destroy the empty_tree /
 !> deallocate(vtree) of run.f90:132-138.  qv_vjp / qv_jvp (no counterpart in
 !> the reference) give the adjoint and tangent of the direct-sum velocity.
 !> Default reals of a -fdefault-real-8 build are real(c_double); default
@@ -25,7 +27,8 @@ module qv_gpu
   integer(c_int32_t), parameter, public :: QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2
   integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
                                        QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
-                                       QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6
+                                       QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6, &
+                                       QV_ERR_LINE_EMPTY = 7, QV_ERR_CAPACITY = 8
 
   !> run.in parameters of the hot path (cdata.f90:196-416)
   type, bind(C), public :: qv_params
@@ -39,9 +42,26 @@ module qv_gpu
     integer(c_int64_t) :: nodes, evals, visits, leaf_evals, cell_evals, wave_steps
   end type
 
+  !> host arrays of the line operations: c_loc of x(3,capacity) ... , the
+  !> first pcount slots being f(:)
+  type, bind(C), public :: qv_line_state
+    integer(c_int32_t) :: pcount, capacity
+    type(c_ptr) :: x, u, u1, u2, u3, ghosti, ghostb, ghostii, ghostbb
+    type(c_ptr) :: infront, behind, closest, closestd
+  end type
+
+  !> recon_count, remove_count, small_loop_remove_count, total_length,
+  !> avg_sep, small_loop_remove_length of cdata.f90:146-154, 314-315
+  type, bind(C), public :: qv_line_counts
+    integer(c_int64_t) :: inserted = 0, recon_count = 0, remove_count = 0
+    integer(c_int64_t) :: small_loop_remove_count = 0
+    real(c_double) :: total_length = 0, avg_sep = 0, small_loop_remove_length = 0
+  end type
+
   public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
             qv_velocity, qv_ab_step, qv_rk3_step, qv_enforce_periodic, qv_step, qv_download, &
             qv_tree_dump, qv_closest, qv_probe_velocity, qv_vjp, qv_jvp, qv_get_timings, &
+            qv_download_ghosts, qv_line_pinsert, qv_line_precon, qv_line_premove, &
             qv_status_string, qv_check
 
   interface
@@ -131,6 +151,30 @@ module qv_gpu
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: v(3, *)
       real(c_double), intent(out) :: jv(3, *)
+    end function
+    !> ghost zones of the last ghostp, each x(3,pcount)-shaped (c_null_ptr skips)
+    integer(c_int) function qv_download_ghosts(ctx, gi, gb, gii, gbb) &
+        bind(C, name='qv_download_ghosts')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gi, gb, gii, gbb
+    end function
+    integer(c_int) function qv_line_pinsert(params, s, c) bind(C, name='qv_line_pinsert')
+      import :: c_int, qv_params, qv_line_state, qv_line_counts
+      type(qv_params), intent(in) :: params
+      type(qv_line_state), intent(inout) :: s
+      type(qv_line_counts), intent(inout) :: c
+    end function
+    integer(c_int) function qv_line_precon(params, s, c) bind(C, name='qv_line_precon')
+      import :: c_int, qv_params, qv_line_state, qv_line_counts
+      type(qv_params), intent(in) :: params
+      type(qv_line_state), intent(inout) :: s
+      type(qv_line_counts), intent(inout) :: c
+    end function
+    integer(c_int) function qv_line_premove(params, s, c) bind(C, name='qv_line_premove')
+      import :: c_int, qv_params, qv_line_state, qv_line_counts
+      type(qv_params), intent(in) :: params
+      type(qv_line_state), intent(inout) :: s
+      type(qv_line_counts), intent(inout) :: c
     end function
     integer(c_int) function qv_get_timings(ctx, t) bind(C, name='qv_get_timings')
       import :: c_int, c_ptr, qv_timings

@@ -13,11 +13,16 @@
 !> A data/var.dat present at start-up restarts the run from it as
 !> init_setup does (initial.f90:168-169, data_restore initial.f90:500-529):
 !> the loop continues at the dumped itime + 1 (at 1 with restart_rewind T).
-!> Line operations (pinsert, precon, premove) are not part of this driver.
+!> With line_ops T (this driver's own label, default F) every step runs the
+!> line operations of run.f90:60-72 after the motion: pinsert, and when
+!> recon_shots divides itime the neighbour search on the device tree,
+!> precon (recon_type 'original') and premove; they run on the host through
+!> the library (qv_line_*), the state going down and back up around them.
 !>
 !> run.in labels used: nsteps shots dt delta box_size velocity boundary
 !> quant_circ corea tree_theta tree_extra_correction deriv_order runge_kutta initf pcount
-!> tree_print restart_rewind and gpu_precision (fp64 | fp32c, this driver's own).
+!> tree_print restart_rewind recon_shots, and gpu_precision (fp64 | fp32c) and
+!> line_ops (T | F), this driver's own.
 !>
 !> var.dat holds f(:) as the reference's type qvort (cdata.f90:20-38); the
 !> components this driver does not carry (x_interp, acc, u_mf, u_s_LI,
@@ -35,8 +40,12 @@ program qv_run
   implicit none
   real(c_double), parameter :: pi = 3.14159265358979324_c_double
   ! f(:) arrays
+  ! arrays hold capacity >= pcount slots; the first pcount are f(:)
   real(c_double), allocatable, target :: x(:,:), u(:,:), u1(:,:), u2(:,:), u3(:,:)
   integer(c_int32_t), allocatable, target :: infront(:), behind(:)
+  ! ghost zones and neighbour-search results, for the line operations
+  real(c_double), allocatable, target :: gi(:,:), gb(:,:), gii(:,:), gbb(:,:), closestd(:)
+  integer(c_int32_t), allocatable, target :: closest(:)
   integer(c_int32_t) :: pcount
   ! run.in
   integer :: nsteps = 1, shots = 1, init_pcount = 100
@@ -46,7 +55,9 @@ program qv_run
   logical :: tree_extra_correction = .true., runge_kutta = .false.
   character(len=30) :: velocity = 'Tree', boundary = 'periodic', deriv_order = 'second'
   character(len=30) :: initf = 'single_loop', gpu_precision = 'fp64'
-  logical :: tree_print = .false., restart_rewind = .false.
+  logical :: tree_print = .false., restart_rewind = .false., line_ops = .false.
+  integer :: recon_shots = 1
+  type(qv_line_counts) :: counts
   ! type qvort (cdata.f90:20-38), component for component: the record layout
   ! of f(:) in data/var.dat
   type :: qvort_record
@@ -67,7 +78,6 @@ program qv_run
     real(c_double) :: delta
     real(c_double) :: t_recon(2)
   end type
-  integer(c_int32_t) :: recon_count = 0
   integer :: nstart = 1
   ! state
   type(qv_params) :: p
@@ -117,7 +127,7 @@ program qv_run
   if (have_hist) call qv_check(qv_upload_history(ctx, c_loc(u1), c_loc(u2), c_loc(u3)), &
                                'qv_upload_history')
   call execute_command_line('mkdir -p data')
-  write(*,'(a,i0,a,a,a,a)') ' qv_run: ', count(infront > 0), ' points, velocity ', &
+  write(*,'(a,i0,a,a,a,a)') ' qv_run: ', count(infront(1:pcount) > 0), ' points, velocity ', &
         trim(velocity), ', precision ', trim(gpu_precision)
 
   do itime = nstart, nsteps
@@ -136,6 +146,7 @@ program qv_run
         call print_tree_mesh(itime / shots)
       call qv_check(qv_ab_step(ctx), 'qv_ab_step')
     end if
+    if (line_ops) call line_step()
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
     t = t + dt
     if (mod(itime, shots) == 0) then
@@ -210,6 +221,10 @@ contains
           read(buffer, *, iostat=ios) tree_print
         case ('restart_rewind')
           read(buffer, *, iostat=ios) restart_rewind
+        case ('recon_shots')
+          read(buffer, *, iostat=ios) recon_shots
+        case ('line_ops')
+          read(buffer, *, iostat=ios) line_ops
       end select
       ios = 0
     end do
@@ -250,10 +265,10 @@ contains
              action='read')
         read(fh) version, pcount, nst, flags, nsample
         call alloc_f()
-        read(fh) x
-        read(fh) infront
-        read(fh) behind
-        read(fh) u1, u2, u3
+        read(fh) x(:, 1:pcount)
+        read(fh) infront(1:pcount)
+        read(fh) behind(1:pcount)
+        read(fh) u1(:, 1:pcount), u2(:, 1:pcount), u3(:, 1:pcount)
         close(fh)
         have_hist = any(u1 /= 0._c_double) .or. any(u2 /= 0._c_double) .or. &
                     any(u3 /= 0._c_double)
@@ -265,11 +280,12 @@ contains
   !> data_restore's reads (initial.f90:506-525); quasi particles and
   !> particles are not part of this driver and must be absent
   subroutine data_restore()
-    integer :: fh, dummy_itime, quasi_pcount, part_count
+    integer :: fh, dummy_itime, quasi_pcount, part_count, recon_count
     type(qvort_record), allocatable :: rec(:)
     open(newunit=fh, file='./data/var.dat', form='unformatted', status='old', action='read')
     read(fh) pcount
     read(fh) recon_count
+    counts%recon_count = recon_count
     read(fh) dummy_itime
     read(fh) t
     allocate(rec(pcount))
@@ -335,7 +351,7 @@ contains
     end do
     open(newunit=fh, file='./data/var.dat', form='unformatted', status='replace')
     write(fh) pcount
-    write(fh) recon_count
+    write(fh) int(counts%recon_count, c_int32_t)
     write(fh) itime
     write(fh) t
     write(fh) rec
@@ -368,19 +384,85 @@ contains
   end subroutine
 
   subroutine alloc_f()
-    allocate(x(3, pcount), u(3, pcount), u1(3, pcount), u2(3, pcount), u3(3, pcount))
-    allocate(infront(pcount), behind(pcount))
+    integer :: cap
+    cap = pcount
+    if (line_ops) cap = 2 * pcount + 64
+    allocate(x(3, cap), u(3, cap), u1(3, cap), u2(3, cap), u3(3, cap))
+    allocate(gi(3, cap), gb(3, cap), gii(3, cap), gbb(3, cap))
+    allocate(infront(cap), behind(cap), closest(cap), closestd(cap))
     x = 0._c_double; u = 0._c_double; u1 = 0._c_double; u2 = 0._c_double; u3 = 0._c_double
-    infront = 0; behind = 0
+    gi = 0._c_double; gb = 0._c_double; gii = 0._c_double; gbb = 0._c_double
+    infront = 0; behind = 0; closest = 0; closestd = 0._c_double
   end subroutine
 
-  !> the ts.log line of print_info (output.f90:55-77): columns we do not
-  !> compute here (recon count, maxdu, curvature, removals) are 0
+  !> room for pinsert: one new point per slot at most, so 2 pcount slots
+  subroutine ensure_capacity()
+    integer :: cap, old
+    real(c_double), allocatable :: t3(:,:), t1(:)
+    integer(c_int32_t), allocatable :: ti(:)
+    old = size(infront)
+    if (old >= 2 * pcount) return
+    cap = 2 * pcount + 64
+    call grow3(x); call grow3(u); call grow3(u1); call grow3(u2); call grow3(u3)
+    call grow3(gi); call grow3(gb); call grow3(gii); call grow3(gbb)
+    call growi(infront); call growi(behind); call growi(closest)
+    allocate(t1(cap)); t1 = 0._c_double; t1(1:old) = closestd(1:old)
+    call move_alloc(t1, closestd)
+  contains
+    subroutine grow3(a)
+      real(c_double), allocatable, intent(inout) :: a(:,:)
+      allocate(t3(3, cap)); t3 = 0._c_double; t3(:, 1:old) = a(:, 1:old)
+      call move_alloc(t3, a)
+    end subroutine
+    subroutine growi(a)
+      integer(c_int32_t), allocatable, intent(inout) :: a(:)
+      allocate(ti(cap)); ti = 0; ti(1:old) = a(1:old)
+      call move_alloc(ti, a)
+    end subroutine
+  end subroutine
+
+  function line_state() result(ls)
+    type(qv_line_state) :: ls
+    ls%pcount = pcount
+    ls%capacity = size(infront)
+    ls%x = c_loc(x); ls%u = c_loc(u); ls%u1 = c_loc(u1); ls%u2 = c_loc(u2); ls%u3 = c_loc(u3)
+    ls%ghosti = c_loc(gi); ls%ghostb = c_loc(gb); ls%ghostii = c_loc(gii)
+    ls%ghostbb = c_loc(gbb)
+    ls%infront = c_loc(infront); ls%behind = c_loc(behind)
+    ls%closest = c_loc(closest); ls%closestd = c_loc(closestd)
+  end function
+
+  !> run.f90:60-72 after the motion: pinsert, then every recon_shots steps
+  !> pclose_tree / pclose (device), precon and premove (host); the state
+  !> goes back to the device with its AB history
+  subroutine line_step()
+    type(qv_line_state) :: ls
+    call ensure_capacity()
+    call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
+                  'qv_download')
+    call qv_check(qv_download_ghosts(ctx, c_loc(gi), c_loc(gb), c_loc(gii), c_loc(gbb)), &
+                  'qv_download_ghosts')
+    ls = line_state()
+    call qv_check(qv_line_pinsert(p, ls, counts), 'line.mod:pinsert')
+    pcount = ls%pcount
+    if (mod(itime, recon_shots) == 0) then
+      call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
+      call qv_check(qv_closest(ctx, closest, closestd), 'qv_closest')
+      ls = line_state()
+      call qv_check(qv_line_precon(p, ls, counts), 'line.mod:precon')
+      call qv_check(qv_line_premove(p, ls, counts), 'line.mod:premove')
+    end if
+    call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
+    call qv_check(qv_upload_history(ctx, c_loc(u1), c_loc(u2), c_loc(u3)), 'qv_upload_history')
+  end subroutine
+
+  !> the ts.log line of print_info (output.f90:55-77): recon and removed are
+  !> the line operations' counters; maxdu and curv, not computed here, are 0
   subroutine print_info(filenumber)
     integer, intent(in) :: filenumber
     integer :: fh, nact, j, k
     real(c_double) :: length, maxu, d(3)
-    nact = count(infront > 0)
+    nact = count(infront(1:pcount) > 0)
     length = 0._c_double
     maxu = 0._c_double
     do j = 1, pcount
@@ -401,8 +483,8 @@ contains
     if (filenumber == 1) write(fh, *) '%--var--------t-------pcount-------recon----avg_d-----length&
         &--------maxu---------maxdu-----num eval----curv------removed'
     write(fh, '(i6.4,f13.7,i10.1,i13.1,f7.4,f13.6,f13.5,f13.5,f10.2,f10.2,i13.1)') &
-      filenumber, t, nact, 0, length / nact / delta, length, maxu, 0._c_double, &
-      real(evals, c_double) / nact, 0._c_double, 0
+      filenumber, t, nact, int(counts%recon_count), length / nact / delta, length, maxu, &
+      0._c_double, real(evals, c_double) / nact, 0._c_double, int(counts%remove_count)
     close(fh)
   end subroutine
 
@@ -419,13 +501,13 @@ contains
     open(newunit=fh, file=print_file, status='replace', form='unformatted', access='stream')
     write(fh) t
     write(fh) pcount
-    write(fh) x(1, :)
-    write(fh) x(2, :)
-    write(fh) x(3, :)
-    write(fh) infront
-    write(fh) u(1, :)
-    write(fh) u(2, :)
-    write(fh) u(3, :)
+    write(fh) x(1, 1:pcount)
+    write(fh) x(2, 1:pcount)
+    write(fh) x(3, 1:pcount)
+    write(fh) infront(1:pcount)
+    write(fh) u(1, 1:pcount)
+    write(fh) u(2, 1:pcount)
+    write(fh) u(3, 1:pcount)
     write(fh) z, z, z, z, z, z, z, z
     close(fh)
   end subroutine

@@ -191,3 +191,50 @@ def with_holes(fil: Filaments, n_holes: int, seed: int = 7) -> Filaments:
     infront[new_of_old] = new_of_old[fil.infront - 1] + 1
     behind[new_of_old] = new_of_old[fil.behind - 1] + 1
     return Filaments(x, infront, behind, new_of_old[fil.loop_start], fil.loop_len.copy())
+
+
+def reconnection_scene(delta: float = 5e-4, radius: float = 0.01, gap: float = 0.3,
+                       seed: int = 81) -> Filaments:
+    """A state that exercises every line operation of one step.
+
+    Ring A (x-y plane, centre 0) and ring B (x-z plane) pass within
+    gap * delta of each other at (radius, 0, 0), with crossing tangents, so
+    the pair reconnects (closestd < delta/2, line length shrinks).  Ring A
+    has one empty slot on its far side, a gap of two spacings that pinsert
+    fills; ring B has an extra point half a spacing after its far point,
+    which premove removes; a 5-point loop of radius delta/2 (its points two
+    apart are closer than delta) loses a point to premove and is then removed
+    by loop_killer.  Spacing 0.8 delta; positions carry a seeded jitter of
+    1e-3 delta."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 2 * int(round(math.pi * radius / (0.8 * delta)))
+    th = 2.0 * math.pi * np.arange(n) / n
+    ring_a = np.stack([radius * np.cos(th), radius * np.sin(th), np.zeros(n)], axis=1)
+    cx = 2.0 * radius + gap * delta
+    ring_b = np.stack([cx - radius * np.cos(th), np.zeros(n), -radius * np.sin(th)], axis=1)
+    far = n // 2
+    ph = 0.5 * (th[far] + th[far + 1])
+    extra = np.array([[cx - radius * math.cos(ph), 0.0, -radius * math.sin(ph)]])
+    r_small = 0.5 * delta
+    ts = 2.0 * math.pi * np.arange(5) / 5
+    small = np.stack([r_small * np.cos(ts), r_small * np.sin(ts), np.full(5, 0.03)], axis=1)
+    x = np.concatenate([ring_a, ring_b, extra, small])
+    x += rng.normal(scale=1e-3 * delta, size=x.shape)
+    m = x.shape[0]
+    inf = np.zeros(m, dtype=np.int32)
+    beh = np.zeros(m, dtype=np.int32)
+
+    def chain(slots):
+        for k, s in enumerate(slots):
+            inf[s] = slots[(k + 1) % len(slots)] + 1
+            beh[s] = slots[k - 1] + 1
+
+    chain([k for k in range(n) if k != far])
+    b_slots = list(range(n, 2 * n))
+    b_slots.insert(far + 1, 2 * n)
+    chain(b_slots)
+    chain(list(range(2 * n + 1, 2 * n + 6)))
+    x[far] = 0.0
+    starts = np.array([0, n, 2 * n + 1])
+    lens = np.array([n - 1, n + 1, 5])
+    return Filaments(x, inf, beh, starts, lens)

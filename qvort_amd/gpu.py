@@ -27,7 +27,7 @@ BC = {"open": 0, "periodic": 1, "openx": 2, "openxy": 3}
 PREC = {"fp64": 0, "fp32c": 1}
 
 QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
-    QV_ERR_NO_DEVICE = range(7)
+    QV_ERR_NO_DEVICE, QV_ERR_LINE_EMPTY, QV_ERR_CAPACITY = range(9)
 QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE = 1, 2
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
@@ -36,6 +36,7 @@ EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_h
            "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_batch_create",
            "qv_batch_destroy", "qv_batch_upload", "qv_batch_velocity", "qv_batch_step",
            "qv_batch_download", "qv_batch_tune", "qv_batch_set_variant", "qv_batch_info",
+           "qv_download_ghosts", "qv_line_pinsert", "qv_line_precon", "qv_line_premove",
            "qv_status_string"]
 
 
@@ -52,6 +53,21 @@ class QvTimings(C.Structure):
                 ("total_ms", C.c_float), ("nodes", C.c_int64), ("evals", C.c_int64),
                 ("visits", C.c_int64), ("leaf_evals", C.c_int64), ("cell_evals", C.c_int64),
                 ("wave_steps", C.c_int64)]
+
+
+class QvLineState(C.Structure):
+    _fields_ = [("pcount", C.c_int32), ("capacity", C.c_int32)] + \
+        [(n, C.POINTER(C.c_double)) for n in ("x", "u", "u1", "u2", "u3", "ghosti", "ghostb",
+                                               "ghostii", "ghostbb")] + \
+        [(n, C.POINTER(C.c_int32)) for n in ("infront", "behind", "closest")] + \
+        [("closestd", C.POINTER(C.c_double))]
+
+
+class QvLineCounts(C.Structure):
+    _fields_ = [("inserted", C.c_int64), ("recon_count", C.c_int64),
+                ("remove_count", C.c_int64), ("small_loop_remove_count", C.c_int64),
+                ("total_length", C.c_double), ("avg_sep", C.c_double),
+                ("small_loop_remove_length", C.c_double)]
 
 
 class QvortError(RuntimeError):
@@ -129,6 +145,9 @@ def lib():
         L.qv_batch_tune.argtypes = [vp, C.c_int32, i32, P(C.c_float)]
         L.qv_batch_set_variant.argtypes = [vp, C.c_int32]
         L.qv_batch_info.argtypes = [vp, i32, i32, i32, i32, P(C.c_float)]
+        L.qv_download_ghosts.argtypes = [vp, d, d, d, d]
+        for name in ("qv_line_pinsert", "qv_line_precon", "qv_line_premove"):
+            getattr(L, name).argtypes = [P(QvParams), P(QvLineState), P(QvLineCounts)]
         L.qv_status_string.argtypes = [C.c_int]
         L.qv_status_string.restype = C.c_char_p
         for name in EXPORTS[:-1]:
@@ -236,6 +255,13 @@ class Context:
         n = self.pcount
         arrs = [np.zeros((n, 3)) for _ in range(5)]
         _check(lib().qv_download(self._h, *[_dp(a) for a in arrs]), "qv_download")
+        return tuple(arrs)
+
+    def download_ghosts(self):
+        """(ghosti, ghostb, ghostii, ghostbb) of the last ghostp."""
+        n = self.pcount
+        arrs = [np.zeros((n, 3)) for _ in range(4)]
+        _check(lib().qv_download_ghosts(self._h, *[_dp(a) for a in arrs]), "qv_download_ghosts")
         return tuple(arrs)
 
     def tree_dump(self) -> np.ndarray:

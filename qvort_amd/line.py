@@ -1,0 +1,116 @@
+"""The line operations of a step (run.f90:60-72) on numpy arrays, through
+the library's host entry points qv_line_pinsert / qv_line_precon /
+qv_line_premove (include/qvort_gpu.h).
+
+:class:`LineState` holds the f(:) fields they read and write, with spare
+slots for the points pinsert appends; :func:`topology_step` runs a step's
+line operations in the reference's order with the neighbour search on the
+device tree of the step's velocity stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import gpu
+
+VEC_FIELDS = ("x", "u", "u1", "u2", "u3", "ghosti", "ghostb", "ghostii", "ghostbb")
+INT_FIELDS = ("infront", "behind", "closest")
+
+
+class LineState:
+    """Host arrays of `capacity` slots, of which the first `pcount` are f(:)."""
+
+    def __init__(self, pcount: int, capacity: int | None = None):
+        cap = max(pcount, capacity if capacity is not None else pcount + pcount // 8 + 64)
+        self.pcount = int(pcount)
+        self.capacity = int(cap)
+        for n in VEC_FIELDS:
+            setattr(self, "_" + n, np.zeros((cap, 3)))
+        for n in INT_FIELDS:
+            setattr(self, "_" + n, np.zeros(cap, dtype=np.int32))
+        self._closestd = np.zeros(cap)
+
+    @classmethod
+    def from_arrays(cls, capacity: int | None = None, **arrays) -> "LineState":
+        n = np.asarray(arrays["x"]).shape[0]
+        st = cls(n, capacity)
+        for name, a in arrays.items():
+            if a is not None:
+                getattr(st, "_" + name)[:n] = a
+        return st
+
+    def __getattr__(self, name):
+        if name in VEC_FIELDS or name in INT_FIELDS or name == "closestd":
+            return self.__dict__["_" + name][:self.pcount]
+        raise AttributeError(name)
+
+    def _struct(self) -> gpu.QvLineState:
+        st = gpu.QvLineState()
+        st.pcount = self.pcount
+        st.capacity = self.capacity
+        for n in VEC_FIELDS + ("closestd",):
+            setattr(st, n, getattr(self, "_" + n).ctypes.data_as(C.POINTER(C.c_double)))
+        for n in INT_FIELDS:
+            setattr(st, n, getattr(self, "_" + n).ctypes.data_as(C.POINTER(C.c_int32)))
+        return st
+
+    def _call(self, fn: str, p: gpu.QvParams, counts: gpu.QvLineCounts) -> None:
+        st = self._struct()
+        rc = getattr(gpu.lib(), fn)(C.byref(p), C.byref(st), C.byref(counts))
+        self.pcount = st.pcount
+        gpu._check(rc, fn)
+
+
+def new_counts() -> gpu.QvLineCounts:
+    return gpu.QvLineCounts()
+
+
+def counts_dict(c: gpu.QvLineCounts) -> dict:
+    return {n: getattr(c, n) for n, _ in gpu.QvLineCounts._fields_}
+
+
+def pinsert(p: gpu.QvParams, st: LineState, counts: gpu.QvLineCounts) -> None:
+    """pinsert (line.f90:18-108)."""
+    st._call("qv_line_pinsert", p, counts)
+
+
+def precon(p: gpu.QvParams, st: LineState, counts: gpu.QvLineCounts) -> None:
+    """precon_original (reconnection.f90:145-256) with loop_killer."""
+    st._call("qv_line_precon", p, counts)
+
+
+def premove(p: gpu.QvParams, st: LineState, counts: gpu.QvLineCounts) -> None:
+    """premove (line.f90:114-161) with loop_killer."""
+    st._call("qv_line_premove", p, counts)
+
+
+def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
+                  counts: gpu.QvLineCounts, *, search: bool) -> None:
+    """The line operations after a step's velocity and AB update
+    (run.f90:60-72): the state and the step's ghosts come back from the
+    device, pinsert runs on the host, and when `search` is set (recon_shots
+    divides itime) the device searches the step's tree for every slot's
+    closest point before precon and premove run on the host.  The result is
+    uploaded with its AB history; the caller then wraps (qv_enforce_periodic)
+    as the reference does after its ghostp.  The host owns the topology: `st`
+    holds the current links (and closest/closestd) between calls; the device
+    sends back positions, velocities and ghosts only."""
+    n = ctx.pcount
+    x, u, u1, u2, u3 = ctx.download()
+    gi, gb, gii, gbb = ctx.download_ghosts()
+    st.pcount = n
+    for name, a in (("x", x), ("u", u), ("u1", u1), ("u2", u2), ("u3", u3), ("ghosti", gi),
+                    ("ghostb", gb), ("ghostii", gii), ("ghostbb", gbb)):
+        getattr(st, "_" + name)[:n] = a
+    pinsert(p, st, counts)
+    if search:
+        ctx.upload(st.x, st.infront, st.behind)
+        cl, cd = ctx.closest(st.closest, st.closestd)
+        st.closest[:] = cl
+        st.closestd[:] = cd
+        precon(p, st, counts)
+        premove(p, st, counts)
+    ctx.upload(st.x, st.infront, st.behind)
+    ctx.upload_history(st.u1, st.u2, st.u3)

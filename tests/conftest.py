@@ -26,6 +26,11 @@ def closest_names():
         return sorted(json.load(fh))
 
 
+def line_names():
+    with open(os.path.join(GOLDEN, "line_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
 def fmt_names():
     with open(os.path.join(GOLDEN, "fmt_index.json")) as fh:
         return sorted(json.load(fh))

@@ -10,6 +10,7 @@ case records the run.in parameters it used.  Run from the repository root:
     python tests/golden/make_golden.py --probe      # velocity at probe points
     python tests/golden/make_golden.py --rk3        # pmotion_RK3 trajectories
     python tests/golden/make_golden.py --formats    # var.dat and tree_mesh files
+    python tests/golden/make_golden.py --line       # pinsert / precon / premove
 
 The neighbour-search cases (closest_*.npz, index in closest_index.json) run
 one step of ghostp, construct_tree, pmotion, optionally pinsert, then
@@ -187,6 +188,53 @@ def main_rk3():
         json.dump(index, fh, indent=1, sort_keys=True)
 
 
+def line_cases():
+    scene = G.reconnection_scene(seed=81)
+    scene2 = G.reconnection_scene(seed=82, gap=0.2)
+    tangle = G.random_loops(0.03, 5e-4, n_loops=6, seed=83)
+    base = dict(delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4, box_size=0.1)
+    return [
+        ("line_scene_tree_periodic", scene, dict(base, velocity="Tree", tree_theta=0.4,
+                                                 boundary="periodic", recon_shots=1), 3),
+        ("line_scene_bs_open", scene2, dict(base, velocity="BS", tree_theta=0.0,
+                                            boundary="open", recon_shots=1), 3),
+        ("line_tangle_tree_periodic", tangle, dict(base, velocity="Tree", tree_theta=0.4,
+                                                   boundary="periodic", box_size=0.03,
+                                                   recon_shots=1), 3),
+        ("line_scene_fourth_openx", scene, dict(base, velocity="Tree", tree_theta=0.5,
+                                                boundary="openx", deriv_order="fourth",
+                                                recon_shots=2), 4),
+        # spacing 1.05 delta everywhere and no empty slot: pinsert appends a
+        # slot for every segment
+        ("line_sparse_ring_bs_periodic", G.single_ring(60, 60 * 1.05 * 5e-4 / (2 * np.pi)),
+         dict(base, velocity="BS", tree_theta=0.0, boundary="periodic", recon_shots=1), 2),
+    ]
+
+
+def main_line():
+    index = {}
+    for name, fil, params, nsteps in line_cases():
+        r = refrun.line_run(fil, nsteps=nsteps, threads=1, **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind)
+        stages = [st for st in refrun.LINE_STAGES if st in r]
+        for st in stages:
+            for k, v in r[st].items():
+                arrays[f"{st}__{k}"] = np.asarray(v)
+        fin = r["final"]
+        for k in ("pcount", "x", "u1", "u2", "infront", "behind"):
+            arrays[f"final__{k}"] = np.asarray(fin[k])
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount, stages=stages,
+                    counters1=r["counters1"],
+                    counters_final={k: fin[k] for k in r["counters1"]})
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "->", fin["pcount"], r["counters1"], meta["counters_final"],
+              flush=True)
+    with open(os.path.join(OUT, "line_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def fmt_cases():
     holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=3, seed=71), 20, seed=7)
     loops = G.random_loops(0.1, 5e-4, n_loops=2, seed=72)
@@ -225,6 +273,9 @@ def main_formats():
 
 
 def main():
+    if "--line" in sys.argv[1:]:
+        main_line()
+        return
     if "--formats" in sys.argv[1:]:
         main_formats()
         return

@@ -1,0 +1,91 @@
+"""A run with line operations on the GPU path: velocity and AB update on the
+device, pinsert / precon / premove on the host, the neighbour search on the
+device tree (qvort_amd.line.topology_step), against the reference build's
+run of the same steps (tests/golden/line_*.npz): links, slot count and
+counters exactly, positions to 1e-15 cm, AB history to 1e-12."""
+import numpy as np
+import pytest
+
+from conftest import line_names, load_golden
+
+pytestmark = pytest.mark.gpu
+
+NAMES = line_names()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_run_with_line_operations(name):
+    from qvort_amd import gpu, line
+    g = load_golden(name)
+    meta = g["meta"]
+    pr = dict(meta["params"])
+    recon_shots = pr.pop("recon_shots")
+    p = gpu.params(**pr)
+    print("case", name, flush=True)
+    st = line.LineState.from_arrays(capacity=2 * meta["pcount"] + 8, x=g["x0"],
+                                    infront=g["infront"], behind=g["behind"])
+    c = line.new_counts()
+    with gpu.Context(p) as ctx:
+        ctx.upload(g["x0"], g["infront"], g["behind"])
+        for itime in range(1, meta["nsteps"] + 1):
+            ctx.velocity_only()
+            ctx.ab_step()
+            line.topology_step(ctx, p, st, c, search=itime % recon_shots == 0)
+            ctx.enforce_periodic()
+        x, _, u1, u2, _ = ctx.download()
+    n = int(g["final__pcount"])
+    assert st.pcount == n and ctx.pcount == n
+    assert np.array_equal(st.infront, g["final__infront"])
+    assert np.array_equal(st.behind, g["final__behind"])
+    act = g["final__infront"] != 0
+    assert np.max(np.abs(x - g["final__x"])[act]) <= 1e-15
+    for k, ref in ((1, g["final__u1"]), (2, g["final__u2"])):
+        got = (u1 if k == 1 else u2)[act]
+        nr = np.linalg.norm(ref[act], axis=1)
+        m = nr > 0
+        assert np.max(np.linalg.norm(got - ref[act], axis=1)[m] / nr[m]) <= 1e-12
+    ref = meta["counters_final"]
+    assert c.recon_count == ref["recon_count"]
+    assert c.remove_count == ref["remove_count"]
+    assert c.small_loop_remove_count == ref["small_loop_remove_count"]
+    assert abs(c.total_length - ref["total_length"]) <= 1e-12 * ref["total_length"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_fortran_driver_with_line_operations(name, tmp_path):
+    import os
+    import subprocess
+
+    from conftest import ROOT
+    from qvort_amd import formats
+    g = load_golden(name)
+    meta = g["meta"]
+    p = meta["params"]
+    n, ns = meta["pcount"], meta["nsteps"]
+    print("case", name, flush=True)
+    lines = [f"nsteps {ns}", f"shots {ns}", f"dt {p['dt']!r}", f"delta {p['delta']!r}",
+             f"box_size {p['box_size']!r}", f"velocity {p['velocity']}",
+             f"boundary {p['boundary']}", f"quant_circ {p['quant_circ']!r}",
+             f"corea {p['corea']!r}", f"tree_theta {p['tree_theta']!r}",
+             f"deriv_order {p.get('deriv_order', 'second')}",
+             f"recon_shots {p['recon_shots']}", "line_ops T", "initf state_file",
+             "gpu_precision fp64"]
+    (tmp_path / "run.in").write_text("\n".join(lines) + "\n")
+    with open(tmp_path / "state.bin", "wb") as fh:
+        np.array([1, n, ns, 0, 0], dtype="<i4").tofile(fh)
+        g["x0"].astype("<f8").tofile(fh)
+        g["infront"].astype("<i4").tofile(fh)
+        g["behind"].astype("<i4").tofile(fh)
+        np.zeros((3, n, 3)).astype("<f8").tofile(fh)
+    r = subprocess.run([os.path.join(ROOT, "qvort_amd", "lib", "qv_run")], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    d = formats.read_var_dat(tmp_path / "data" / "var.dat")
+    f = d["f"]
+    assert d["pcount"] == int(g["final__pcount"])
+    assert d["recon_count"] == meta["counters_final"]["recon_count"]
+    assert np.array_equal(f["infront"], g["final__infront"])
+    assert np.array_equal(f["behind"], g["final__behind"])
+    act = g["final__infront"] != 0
+    assert np.max(np.abs(f["x"] - g["final__x"])[act]) <= 1e-15
