@@ -26,8 +26,14 @@
 // VALU), T < 0 is read from the sign of T's high word (a scalar test), and
 // the accepting lanes are M & ballot(d2 >= T).  In FP32C every internal
 // node's multipole is evaluated with weight 0 on the lanes that do not
-// accept: a uniform branch around it costs more (accumulator copies) than
-// the 7% of steps in which no lane accepts.
+// accept: a uniform branch around it saves nothing measurable on the 7% of
+// steps in which no lane accepts.  Each step's term (leaf or cell) is added
+// to the target's sum at one place after the branches, so the sums stay in
+// the same registers on every path (no copies at the join).  The image
+// shells' shift lives in vector registers in FP32C, so that cent + shift
+// reads the node's centroid as the one scalar operand of each add.  The
+// kernel's output pointers are read from the kernel-argument segment after
+// the walk: held in scalar registers through it they would be spilled.
 //
 // QV_FP64: leaf terms in the reference's operation order, multipoles as
 // [circ x r - eps:C] / d^3 + 3 r x (C^T r) / d^5 (the reference's double
@@ -89,7 +95,7 @@ __device__ inline bool accept_full(double d2, double w, double g, double theta, 
 // only where `on` (the other lanes may hold inf/nan terms).
 template <bool kShift>
 __device__ inline void leaf64(bool on, const double xi[3], double cx, double cy, double cz,
-                              double gx, double gy, double gz, const double s[3], double u[3]) {
+                              double gx, double gy, double gz, const double s[3], double t[3]) {
   // x + shift with shift = 0 is x itself, so the central shell skips the add
   const double R1[3] = {(kShift ? cx + s[0] : cx) - xi[0], (kShift ? cy + s[1] : cy) - xi[1],
                         (kShift ? cz + s[2] : cz) - xi[2]};
@@ -99,9 +105,9 @@ __device__ inline void leaf64(bool on, const double xi[3], double cx, double cy,
   const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
   const double c = (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
   const double f = on ? (a + b) / c : 0.;  // finite cross product, so 0 * it = 0
-  u[0] = u[0] + f * (R1[1] * R2[2] - R1[2] * R2[1]);
-  u[1] = u[1] + f * (R1[2] * R2[0] - R1[0] * R2[2]);
-  u[2] = u[2] + f * (R1[0] * R2[1] - R1[1] * R2[0]);
+  t[0] = f * (R1[1] * R2[2] - R1[2] * R2[1]);
+  t[1] = f * (R1[2] * R2[0] - R1[0] * R2[2]);
+  t[2] = f * (R1[0] * R2[1] - R1[1] * R2[0]);
 }
 
 // cell term, tree.f90:400-411 (without Gamma/(4 pi)); r = x_i - (cent +
@@ -118,7 +124,7 @@ __device__ inline double rsqrt64(double x) {
 }
 
 __device__ inline void cell64(bool on, const double r[3], double d2,
-                              const double *__restrict__ m, double u[3]) {
+                              const double *__restrict__ m, double t[3]) {
   // lanes that are off use d2 = 1 (finite terms) and weight 0
   const double id = rsqrt64(on ? d2 : 1.);
   const double id2 = id * id;
@@ -132,9 +138,9 @@ __device__ inline void cell64(bool on, const double r[3], double d2,
   const double v0 = (r[0] * C[0] + r[1] * C[3]) + r[2] * C[6];
   const double v1 = (r[0] * C[1] + r[1] * C[4]) + r[2] * C[7];
   const double v2 = (r[0] * C[2] + r[1] * C[5]) + r[2] * C[8];
-  u[0] = u[0] + (((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5);
-  u[1] = u[1] + (((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5);
-  u[2] = u[2] + (((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5);
+  t[0] = ((c1 * r[2] - c2 * r[1]) - A0) * id3 + (3. * (r[1] * v2 - r[2] * v1)) * id5;
+  t[1] = ((c2 * r[0] - c0 * r[2]) - A1) * id3 + (3. * (r[2] * v0 - r[0] * v2)) * id5;
+  t[2] = ((c0 * r[1] - c1 * r[0]) - A2) * id3 + (3. * (r[0] * v1 - r[1] * v0)) * id5;
 }
 
 // FP32 terms.  The parity code of this file is compiled without
@@ -147,7 +153,7 @@ __device__ inline float fmaf_(float a, float b, float c) { return __builtin_fmaf
 // for targets projecting inside the segment, ab + R1.R2 = |R1 x e|^2 /
 // (ab - R1.R2), which keeps FP32 free of cancellation.
 __device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, double e0d,
-                              double e1d, double e2d, AccF64 &acc) {
+                              double e1d, double e2d, float T[3], float &W) {
   const float r0 = (float)r0d, r1 = (float)r1d, r2 = (float)r2d;
   const float e0 = (float)e0d, e1 = (float)e1d, e2 = (float)e2d;
   const float c0 = fmaf_(r1, e2, -r2 * e1), c1 = fmaf_(r2, e0, -r0 * e2),
@@ -169,24 +175,21 @@ __device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, doubl
   }
   // lanes that are off may hold inf/nan in f (e.g. their own point); R1 x e
   // is finite, so selecting f alone zeroes their contribution
-  const double fd = (double)(on ? f : 0.f);
-  acc.s0 = fma(fd, (double)c0, acc.s0);
-  acc.s1 = fma(fd, (double)c1, acc.s1);
-  acc.s2 = fma(fd, (double)c2, acc.s2);
+  W = on ? f : 0.f;
+  T[0] = c0; T[1] = c1; T[2] = c2;
 }
 
 // FP32 cell: m6 = circ(3), eps:C(3) from the node record, C(9) from mp32.
 // t = [(circ x r - eps:C) + 3 r x (C^T r) / d^2] / d^3, the 1/d^3 applied in
 // the FP64 accumulation.
 __device__ inline void cell32(bool on, const double rd[3], double d2, const float *m6,
-                              const float *__restrict__ C, AccF64 &acc) {
+                              const float *__restrict__ C, float T[3], float &W) {
   const float r0 = (float)rd[0], r1 = (float)rd[1], r2 = (float)rd[2];
   // lanes that are off use d2 = 1 (finite terms) and weight 0
   const float id = __builtin_amdgcn_rsqf(on ? (float)d2 : 1.f);
   const float id2 = id * id;
   const float q = 3.f * id2;
-  const float id3f = on ? id2 * id : 0.f;
-  const double id3 = (double)id3f;
+  W = on ? id2 * id : 0.f;
   const float v0 = fmaf_(r2, C[6], fmaf_(r1, C[3], r0 * C[0]));
   const float v1 = fmaf_(r2, C[7], fmaf_(r1, C[4], r0 * C[1]));
   const float v2 = fmaf_(r2, C[8], fmaf_(r1, C[5], r0 * C[2]));
@@ -196,9 +199,9 @@ __device__ inline void cell32(bool on, const double rd[3], double d2, const floa
   const float Y0 = fmaf_(r1, v2, -r2 * v1);
   const float Y1 = fmaf_(r2, v0, -r0 * v2);
   const float Y2 = fmaf_(r0, v1, -r1 * v0);
-  acc.s0 = fma((double)fmaf_(Y0, q, X0), id3, acc.s0);
-  acc.s1 = fma((double)fmaf_(Y1, q, X1), id3, acc.s1);
-  acc.s2 = fma((double)fmaf_(Y2, q, X2), id3, acc.s2);
+  T[0] = fmaf_(Y0, q, X0);
+  T[1] = fmaf_(Y1, q, X1);
+  T[2] = fmaf_(Y2, q, X2);
 }
 
 struct alignas(16) StackEntry {
@@ -226,6 +229,14 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
   typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
   const char *recb = reinterpret_cast<const char *>(A.rec);
   const char *m32b = reinterpret_cast<const char *>(A.mp32);
+  // the shift in vector registers (a lane shuffle makes it a vector value):
+  // cent + shift then reads one scalar operand, with no copy of the
+  // centroid into vector registers
+  // (FP32C only: the FP64 walk has no vector registers to spare)
+  double s_[3] = {sv[0], sv[1], sv[2]};
+  if (kShift && kMode != QV_FP64) {
+    s_[0] = __shfl(sv[0], 0); s_[1] = __shfl(sv[1], 0); s_[2] = __shfl(sv[2], 0);
+  }
   while (true) {
     // byte offset of record n in 32 bits (64 B per record, 16 floats of
     // mp32 per node: the same offset); nnodes < 2^26
@@ -238,12 +249,16 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
     const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
     wc.nvis += (uint32_t)__popcll(M);
     // r = x_i - (cent + shift), d2 as the reference forms dist (tree.f90:351-353)
-    const double cs0 = kShift ? rec.cx + sv[0] : rec.cx;
-    const double cs1 = kShift ? rec.cy + sv[1] : rec.cy;
-    const double cs2 = kShift ? rec.cz + sv[2] : rec.cz;
+    const double cs0 = kShift ? rec.cx + s_[0] : rec.cx;
+    const double cs1 = kShift ? rec.cy + s_[1] : rec.cy;
+    const double cs2 = kShift ? rec.cz + s_[2] : rec.cz;
     const double r[3] = {xi[0] - cs0, xi[1] - cs1, xi[2] - cs2};
     const double d2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
     uint64_t O = 0;
+    // this step's term, added once after the branches: FP64 t, or the FP32
+    // vector T with weight W
+    double t64[3] = {0., 0., 0.};
+    float T[3] = {0.f, 0.f, 0.f}, W = 0.f;
     if (j >= 0) {
       // leaf: the single point's segment unless it is i, behind(i), infront(i)
       const bool on = mine && j != t && j != bi && j != fi;
@@ -252,10 +267,10 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
         // sqrt(d2) < epsilon(0.) (tree.f90:372) is exactly d2 < 2^-104
         if (__ballot(on && d2 < 0x1p-104)) wc.singular = true;
         if (kMode == QV_FP64) {
-          leaf64<kShift>(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], sv, u);
+          leaf64<kShift>(on, xi, rec.cx, rec.cy, rec.cz, rec.g[0], rec.g[1], rec.g[2], s_, t64);
         } else {
           leaf32(on, -r[0], -r[1], -r[2], rec.g[0] - rec.cx, rec.g[1] - rec.cy,
-                 rec.g[2] - rec.cz, acc);
+                 rec.g[2] - rec.cz, T, W);
         }
         wc.nleaf += (uint32_t)__popcll(L);
       }
@@ -270,12 +285,20 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
       }
       const bool ok = __builtin_amdgcn_inverse_ballot_w64(Acc_);
       if (kMode == QV_FP64) {
-        if (Acc_) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, u);
+        if (Acc_) cell64(ok, r, d2, A.mp + 12 * (int64_t)n, t64);
       } else {
-        cell32(ok, r, d2, rec.m6, reinterpret_cast<const float *>(m32b + off), acc);
+        cell32(ok, r, d2, rec.m6, reinterpret_cast<const float *>(m32b + off), T, W);
       }
       wc.ncell += (uint32_t)__popcll(Acc_);
       O = M & ~Acc_;
+    }
+    if (kMode == QV_FP64) {
+      u[0] = u[0] + t64[0]; u[1] = u[1] + t64[1]; u[2] = u[2] + t64[2];
+    } else {
+      const double w = (double)W;
+      acc.s0 = fma(w, (double)T[0], acc.s0);
+      acc.s1 = fma(w, (double)T[1], acc.s1);
+      acc.s2 = fma(w, (double)T[2], acc.s2);
     }
     if (O != 0 && n + 1 != skip) {
       // descend into the children [n+1, skip) with the opening lanes
@@ -300,9 +323,9 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
   }
 }
 
-// 8 wavefronts per SIMD: the register budget that leaves (48 VGPRs, 78
-// SGPRs, no spills in the node loop) hides more of the node loads' latency
-// than 7 waves with spare registers
+// 8 wavefronts per SIMD: the register budget that leaves (53 VGPRs, 78
+// SGPRs, no spills in the FP32C node loop) hides more of the node loads'
+// latency than 7 waves with spare registers
 template <int kMode>
 __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
@@ -344,14 +367,18 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
     const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
     walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, lane, st, u, acc, wc);
   }
+  // the output pointers are read from the kernel-argument segment here, after
+  // the walk, so that they do not occupy scalar registers during it
+  typedef const WalkArgs __attribute__((address_space(4))) KernArgs;
+  KernArgs &O = *(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   // interaction, visit, leaf and step counts (64-bit) and the error flag
   if (lane == 0) {
-    atomicAdd(A.evals + 0, (unsigned long long)wc.nleaf + wc.ncell);
-    atomicAdd(A.evals + 1, (unsigned long long)wc.nvis);
-    atomicAdd(A.evals + 2, (unsigned long long)wc.nleaf);
-    atomicAdd(A.evals + 3, (unsigned long long)wc.nstep);
+    atomicAdd(O.evals + 0, (unsigned long long)wc.nleaf + wc.ncell);
+    atomicAdd(O.evals + 1, (unsigned long long)wc.nvis);
+    atomicAdd(O.evals + 2, (unsigned long long)wc.nleaf);
+    atomicAdd(O.evals + 3, (unsigned long long)wc.nstep);
   }
-  if (wc.singular) atomicOr(A.flags, kFlagSingular);
+  if (wc.singular) atomicOr(O.flags, kFlagSingular);
   if (!valid) return;
   double us[3] = {0., 0., 0.};
   if (kMode == QV_FP64) {
@@ -361,18 +388,18 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   }
   // u_bs * quant_circ / 4 / pi (tree.f90:390, 411), once per sum
   for (int k = 0; k < 3; ++k) {
-    uc[k] = ((uc[k] * A.qc) / 4.) / kPi;
-    us[k] = ((us[k] * A.qc) / 4.) / kPi;
+    uc[k] = ((uc[k] * O.qc) / 4.) / kPi;
+    us[k] = ((us[k] * O.qc) / 4.) / kPi;
   }
   // calc_velocity composition (timestep.f90:273-288)
-  double u0 = A.ulx[t] + uc[0], u1 = A.uly[t] + uc[1], u2 = A.ulz[t] + uc[2];
+  double u0 = O.ulx[t] + uc[0], u1 = O.uly[t] + uc[1], u2 = O.ulz[t] + uc[2];
   double b0 = uc[0], b1 = uc[1], b2 = uc[2];
   if (S.n > 1) {
     u0 = u0 + us[0]; u1 = u1 + us[1]; u2 = u2 + us[2];
     b0 = b0 + us[0]; b1 = b1 + us[1]; b2 = b2 + us[2];
   }
-  A.ux[t] = u0; A.uy[t] = u1; A.uz[t] = u2;
-  A.bx[t] = b0; A.by[t] = b1; A.bz[t] = b2;
+  O.ux[t] = u0; O.uy[t] = u1; O.uz[t] = u2;
+  O.bx[t] = b0; O.by[t] = b1; O.bz[t] = b2;
 }
 
 }  // namespace
