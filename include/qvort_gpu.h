@@ -50,7 +50,7 @@ enum qv_boundary { QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, QV_BC_OPENX = 2, QV_BC_OP
 /* arithmetic of the nonlocal sum */
 enum qv_precision { QV_FP64 = 0, QV_FP32C = 1 };
 /* flags of qv_probe_velocity */
-enum qv_probe_flags { QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2 };
+enum qv_probe_flags { QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2, QV_PROBE_EMPTY_CORE = 4 };
 
 enum qv_status {
   QV_OK = 0,
@@ -173,6 +173,10 @@ int qv_rk3_step(qv_ctx *ctx);
  * adds leaves as u/(4 pi) without Gamma and cells without Gamma/(4 pi)
  * (tree.f90:460, 479); by default both get Gamma/(4 pi), the velocity of
  * the BS sum, and QV_PROBE_REFERENCE_SCALE keeps the reference's factors.
+ * QV_PROBE_EMPTY_CORE (BS only) applies velocity_quasip's empty core
+ * (hamiltonian.f90:42-58): a probe closer than corea to the position of any
+ * slot -- empty slots included, as the reference loops over all of f(:) --
+ * gets u = 0 and a zero gradient.
  * QV_ERR_STATE: no ghosts (BS) or no tree (Tree) yet.  No counterpart for
  * the gradient in the reference. */
 int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, double *grad,

@@ -503,11 +503,14 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
 int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, double *grad,
                       int32_t flags) {
   if (!ctx || nprobe < 0 || (nprobe > 0 && (!y || !u))) return QV_ERR_ARG;
-  if (flags & ~(QV_PROBE_GHOSTP | QV_PROBE_REFERENCE_SCALE)) return QV_ERR_ARG;
+  if (flags & ~(QV_PROBE_GHOSTP | QV_PROBE_REFERENCE_SCALE | QV_PROBE_EMPTY_CORE))
+    return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (c->pcount == 0) return QV_ERR_STATE;
   const bool tree = c->p.velocity == QV_VEL_TREE;
-  if (tree && grad) return QV_ERR_ARG;
+  // velocity_quasip has no Tree form (hamiltonian.f90:49-51, 85-86): the
+  // gradient and the empty core are BS only
+  if (tree && (grad || (flags & QV_PROBE_EMPTY_CORE))) return QV_ERR_ARG;
   if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
   if (flags & QV_PROBE_GHOSTP) {
     TRY(launch_ghost_lia(*c));
@@ -524,7 +527,8 @@ int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, d
   TRY(hipMemcpyAsync(d_y, y, 3 * (size_t)nprobe * sizeof(double), hipMemcpyHostToDevice,
                      c->stream));
   TRY(launch_probe(*c, nprobe, d_y, d_u, grad ? d_g : nullptr, tree,
-                   (flags & QV_PROBE_REFERENCE_SCALE) != 0));
+                   (flags & QV_PROBE_REFERENCE_SCALE) != 0,
+                   (flags & QV_PROBE_EMPTY_CORE) != 0));
   TRY(hipMemcpyAsync(u, d_u, 3 * (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost,
                      c->stream));
   if (grad)

@@ -171,6 +171,6 @@ hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 hipError_t launch_closest(Ctx &c, bool tree);
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
-                        bool tree, bool ref_scale);
+                        bool tree, bool ref_scale, bool empty_core);
 
 }  // namespace qv

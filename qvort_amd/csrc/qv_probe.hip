@@ -31,6 +31,7 @@ struct ProbeArgs {
   int n, nprobe;
   double qc, theta;
   int grad, ref_scale;
+  double core;  // > 0: probes closer than this to any slot's x get u = 0
   const double *X, *Y, *Z, *GX, *GY, *GZ;
   const int32_t *inf;
   const double *py;  // probes, (nprobe,3) row-major
@@ -106,21 +107,30 @@ __global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
     for (int k = 0; k < 3; ++k) y[k] = A.py[3 * q + k];
   double u[3] = {0., 0., 0.};
   double G[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  double md = 1e4;  // squared min_dist = 100 (hamiltonian.f90:44)
   for (int sh = 0; sh < S.n; ++sh) {
     const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
     for (int base = 0; base < A.n; base += kTile) {
       __syncthreads();
       const int j = base + threadIdx.x;
       const bool in = j < A.n && A.inf[j] != 0;
-      sx[threadIdx.x] = in ? A.X[j] : 0.;
-      sy[threadIdx.x] = in ? A.Y[j] : 0.;
-      sz[threadIdx.x] = in ? A.Z[j] : 0.;
+      // every slot's x (empty ones too: the empty-core test reads them);
+      // the terms use only the active ones
+      sx[threadIdx.x] = j < A.n ? A.X[j] : 0.;
+      sy[threadIdx.x] = j < A.n ? A.Y[j] : 0.;
+      sz[threadIdx.x] = j < A.n ? A.Z[j] : 0.;
       gx[threadIdx.x] = in ? A.GX[j] : 0.;
       gy[threadIdx.x] = in ? A.GY[j] : 0.;
       gz[threadIdx.x] = in ? A.GZ[j] : 0.;
       sok[threadIdx.x] = in;
       __syncthreads();
       const int m = act ? min(kTile, A.n - base) : 0;
+      if (A.core > 0. && sh == 0)
+        for (int jj = 0; jj < m; ++jj) {
+          // dist_gen(pos, f(j)%x) over every slot (hamiltonian.f90:45-48)
+          const double d0 = y[0] - sx[jj], d1 = y[1] - sy[jj], d2 = y[2] - sz[jj];
+          md = fmin(md, (d0 * d0 + d1 * d1) + d2 * d2);
+        }
       for (int jj = 0; jj < m; ++jj) {
         if (!sok[jj]) continue;
         const double xj[3] = {sx[jj], sy[jj], sz[jj]};
@@ -130,6 +140,12 @@ __global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
     }
   }
   if (!act) return;
+  // min_dist < corea: the empty core, zero velocity and gradient
+  // (hamiltonian.f90:53-58); min of sqrt = sqrt of min (sqrt is monotone)
+  if (A.core > 0. && sqrt(md) < A.core) {
+    for (int k = 0; k < 3; ++k) u[k] = 0.;
+    for (int k = 0; k < 9; ++k) G[k] = 0.;
+  }
   for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
   if constexpr (kGrad)
     for (int k = 0; k < 9; ++k) A.g[9 * q + k] = G[k];
@@ -212,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S) {
 }  // namespace
 
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
-                        bool tree, bool ref_scale) {
+                        bool tree, bool ref_scale, bool empty_core) {
   ProbeArgs A;
   A.n = c.pcount;
   A.nprobe = nprobe;
@@ -220,6 +236,7 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
   A.theta = c.p.tree_theta;
   A.grad = d_g != nullptr;
   A.ref_scale = ref_scale;
+  A.core = empty_core ? c.p.corea : 0.;
   A.X = c.x.x; A.Y = c.x.y; A.Z = c.x.z;
   A.GX = c.gi.x; A.GY = c.gi.y; A.GZ = c.gi.z;
   A.inf = c.infront;

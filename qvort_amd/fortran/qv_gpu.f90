@@ -24,7 +24,8 @@ module qv_gpu
   integer(c_int32_t), parameter, public :: QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, &
                                            QV_BC_OPENX = 2, QV_BC_OPENXY = 3
   integer(c_int32_t), parameter, public :: QV_FP64 = 0, QV_FP32C = 1
-  integer(c_int32_t), parameter, public :: QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2
+  integer(c_int32_t), parameter, public :: QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2, &
+                                           QV_PROBE_EMPTY_CORE = 4
   integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
                                        QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
                                        QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6, &

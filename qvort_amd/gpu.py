@@ -28,7 +28,7 @@ PREC = {"fp64": 0, "fp32c": 1}
 
 QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
     QV_ERR_NO_DEVICE, QV_ERR_LINE_EMPTY, QV_ERR_CAPACITY = range(9)
-QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE = 1, 2
+QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE, QV_PROBE_EMPTY_CORE = 1, 2, 4
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
            "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic", "qv_step", "qv_download",
@@ -286,9 +286,11 @@ class Context:
         return cl, cd
 
     def probe_velocity(self, y, grad: bool = False, ghostp: bool = False,
-                       reference_scale: bool = False):
+                       reference_scale: bool = False, empty_core: bool = False):
         """Superfluid velocity at probe points y (n,3), as mesh_velocity sums
-        it; with grad=True also du_k/dy_m as an (n,3,3) array (BS only)."""
+        it; with grad=True also du_k/dy_m as an (n,3,3) array (BS only);
+        empty_core=True zeroes both within corea of any slot's position, as
+        velocity_quasip does (BS only)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
         n = y.shape[0]
         if y.shape != (n, 3):
@@ -296,7 +298,8 @@ class Context:
         u = np.zeros((n, 3))
         g = np.zeros((n, 3, 3)) if grad else None
         flags = (QV_PROBE_GHOSTP if ghostp else 0) | \
-            (QV_PROBE_REFERENCE_SCALE if reference_scale else 0)
+            (QV_PROBE_REFERENCE_SCALE if reference_scale else 0) | \
+            (QV_PROBE_EMPTY_CORE if empty_core else 0)
         _check(lib().qv_probe_velocity(self._h, n, _dp(y), _dp(u), _dp(g), flags),
                "qv_probe_velocity")
         return (u, g) if grad else u

@@ -147,3 +147,41 @@ def test_probe_state_and_argument_errors():
         with pytest.raises(gpu.QvortError) as ei:
             c.probe_velocity(y, grad=True)
         assert ei.value.status == gpu.QV_ERR_ARG
+
+
+def test_empty_core_rule():
+    """velocity_quasip's empty core (hamiltonian.f90:42-58): within corea of
+    any slot's position -- an empty slot's too, since the reference loops
+    over all of f(:) -- u and the gradient are zero; elsewhere the flag
+    changes nothing.  Tree mode has no such rule in the reference and is
+    refused."""
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=4, seed=58), 10, seed=3)
+    a = 1.3e-8
+    kw = dict(velocity="BS", tree_theta=0.0, boundary="periodic", box_size=0.1, corea=a)
+    act = np.flatnonzero(fil.infront)
+    hole = np.flatnonzero(fil.infront == 0)[0]
+    fil.x[hole] = [0.011, -0.007, 0.004]  # an empty slot's (stale) position
+    rng = np.random.Generator(np.random.PCG64(59))
+    dirs = rng.normal(size=(4, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    inside = np.stack([fil.x[act[10]] + 0.5 * a * dirs[0], fil.x[act[200]] + 0.9 * a * dirs[1],
+                       fil.x[hole] + 0.5 * a * dirs[2]])
+    outside = np.stack([fil.x[act[10]] + 2.0 * a * dirs[3],
+                        rng.uniform(-0.04, 0.04, size=3)])
+    y = np.concatenate([inside, outside])
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u0, g0 = c.probe_velocity(y, grad=True)
+        u1, g1 = c.probe_velocity(y, grad=True, empty_core=True)
+    assert not np.any(u1[:3]) and not np.any(g1[:3])
+    assert np.any(u0[:3])
+    assert np.array_equal(u1[3:], u0[3:]) and np.array_equal(g1[3:], g0[3:])
+    with gpu.Context(gpu.params(velocity="Tree", tree_theta=0.4, corea=a)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        with pytest.raises(gpu.QvortError) as ei:
+            c.probe_velocity(y, empty_core=True)
+        assert ei.value.status == gpu.QV_ERR_ARG
