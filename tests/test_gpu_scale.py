@@ -1,0 +1,76 @@
+"""The velocity stage at the benchmark's full size: 1,000,042 points in
+seeded random loops (periodic cube L = 0.2 cm, delta = 5e-4 cm), theta =
+0.4, 27 shells -- the oracle cannot walk every target here in a test's
+time, so it walks a strided sample of 4000 targets on the same positions,
+and the rest is checked through properties that do not depend on size.
+
+Bars: FP64 within 1e-12 of the oracle relative to max(|u|, median |u|)
+(per point relative to |u| itself a few points in a million where ~2500
+terms nearly cancel reach 1.4e-12 -- the two sums' round-off); FP32C
+within 1e-5 of FP64 on the same tree; FP32C and FP64 make the same
+decisions (equal interaction counts); the AB step is finite and moves
+every point by less than a spacing."""
+import numpy as np
+import pytest
+
+from conftest import ROOT  # noqa: F401  (puts the repository on sys.path)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def big():
+    from qvort_amd import geometry as G
+    fil = G.random_loops(0.2, 5e-4, target_points=1_000_000, seed=20261202, spacing=0.8)
+    kw = dict(velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.2,
+              delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6)
+    return fil, kw
+
+
+def test_full_size_fp64_sample_vs_oracle_and_fp32c(big):
+    from oracle import port
+    from qvort_amd import gpu
+    fil, kw = big
+    _say("points", fil.pcount)
+    act = np.flatnonzero(fil.infront)
+    sample = act[::len(act) // 4000][:4000].astype(np.int32)
+    with gpu.Context(gpu.params(precision="fp64", **kw)) as c:
+        c.upload_filaments(fil)
+        u64, _, _, ev64 = c.velocity()
+    with gpu.Context(gpu.params(precision="fp32c", **kw)) as c:
+        c.upload_filaments(fil)
+        u32, _, _, ev32 = c.velocity()
+    assert ev32 == ev64
+    _say("evals", ev64)
+    okw = {k: v for k, v in kw.items() if k != "dt"}
+    u_o, ev_o, _, _ = port.velocity_targets(port.make_params(**okw), fil, sample, nthreads=16)
+    nu = np.linalg.norm(u_o, axis=1)
+    scale = np.maximum(nu, np.median(nu))
+    e = np.linalg.norm(u64[sample] - u_o, axis=1) / scale
+    _say("fp64 vs oracle sample: max", e.max())
+    assert e.max() <= 1e-12
+    n64 = np.linalg.norm(u64[act], axis=1)
+    e32 = np.linalg.norm(u32[act] - u64[act], axis=1) / n64
+    _say("fp32c vs fp64: max", e32.max(), "rms", np.sqrt(np.mean(e32 ** 2)))
+    assert e32.max() <= 1e-5
+
+
+def test_full_size_step_properties(big):
+    from qvort_amd import gpu
+    fil, kw = big
+    with gpu.Context(gpu.params(precision="fp32c", **kw)) as c:
+        c.upload_filaments(fil)
+        c.step(2)
+        x, u, u1, u2, u3 = c.download()
+    act = fil.infront != 0
+    assert np.all(np.isfinite(x[act])) and np.all(np.isfinite(u[act]))
+    # u1 holds the last step's velocity, u2 the one before (pmotion's shift)
+    assert np.array_equal(u1[act], u[act])
+    d = x[act] - fil.x[act]
+    d -= 0.2 * np.round(d / 0.2)  # the wrap moves points by a box length
+    assert np.max(np.linalg.norm(d, axis=1)) < 0.8 * 5e-4
+    assert not np.any(x[~act]) or np.array_equal(x[~act], fil.x[~act])
