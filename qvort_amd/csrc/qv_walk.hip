@@ -37,9 +37,9 @@
 //
 // QV_FP64: leaf terms in the reference's operation order, multipoles as
 // [circ x r - eps:C] / d^3 + 3 r x (C^T r) / d^5 (the reference's double
-// loop regrouped) with 1/d from a refined reciprocal square root (a couple
-// of ulp instead of the correctly rounded sqrt and two divisions), the
-// factor Gamma/(4 pi) applied once per sum.
+// loop regrouped, with fused multiply-adds) with 1/d from a refined
+// reciprocal square root (a couple of ulp instead of the correctly rounded
+// sqrt and two divisions), the factor Gamma/(4 pi) applied once per sum.
 // QV_FP32C: relative vectors in FP64, terms in FP32, per-target sums in FP64.
 #include "qv_internal.h"
 
@@ -125,6 +125,9 @@ __device__ inline double rsqrt64(double x) {
 
 __device__ inline void cell64(bool on, const double r[3], double d2,
                               const double *__restrict__ m, double t[3]) {
+  // the regrouped multipole has no reference rounding to keep: fused
+  // multiply-adds allowed here
+#pragma clang fp contract(fast)
   // lanes that are off use d2 = 1 (finite terms) and weight 0
   const double id = rsqrt64(on ? d2 : 1.);
   const double id2 = id * id;
