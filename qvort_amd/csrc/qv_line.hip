@@ -90,8 +90,10 @@ struct Line {
   // operations never leave behind) is not followed past it.
   void loop_killer(int particle) {
     if (s.infront[particle] == 0) return;
+    // the count only decides whether the loop has fewer than 5 points (the
+    // reference also logs loops of 5-40 points), so it stops at 5
     int counter = 1, next = particle;
-    for (int it = 0; it < s.pcount; ++it) {
+    for (int it = 0; it < s.pcount && counter < 5; ++it) {
       if (s.infront[next] == 0) return;
       next = inf(next);
       if (next == particle) break;
@@ -120,16 +122,18 @@ struct Line {
     if (nact == 0) return QV_ERR_LINE_EMPTY;
     const int old = s.pcount;
     double total = 0.;
+    // the first empty slot (minloc of infront): empty slots are only filled
+    // here, never made, so it only moves forward and one cursor finds it
+    int hole = 0;
     for (int i = 0; i < old; ++i) {
       if (s.infront[i] == 0) continue;
       const double disti = lia_dist(X(i), GI(i));
       total = total + disti;
       // the point's delta factor is 1 (delta_adapt F)
       if (!(disti > 1. * p.delta)) continue;
-      // the first empty slot (minloc of infront), else one more slot
-      int nw = -1;
-      for (int k = 0; k < s.pcount; ++k)
-        if (s.infront[k] == 0) { nw = k; break; }
+      // the first empty slot, else one more slot
+      while (hole < s.pcount && s.infront[hole] != 0) ++hole;
+      int nw = hole < s.pcount ? hole : -1;
       if (nw < 0) {
         if (s.pcount >= s.capacity) return QV_ERR_CAPACITY;
         nw = s.pcount++;
