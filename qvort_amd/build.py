@@ -32,6 +32,9 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    # the ensemble kernel's 512-source tile shapes miss their 4-wave register
+    # budget's occupancy on LDS, by design (qv_batch.hip)
+    "-Wno-pass-failed",
 ]
 
 

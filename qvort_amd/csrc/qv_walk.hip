@@ -222,7 +222,7 @@ struct WalkCounts {
 // called with one shift, timestep.f90:277 / :285)
 template <int kMode, bool kShift>
 __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const double xi[3],
-                                  int t, int bi, int fi, uint64_t active, int lane,
+                                  int t, int bi, int fi, uint64_t active,
                                   StackEntry *st, double u[3], AccF64 &acc, WalkCounts &wc) {
   // current entry (node, end of its sibling list, lanes) in SGPRs; the LDS
   // stack holds only sibling lists left behind when descending
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
     const double z[3] = {0., 0., 0.};
     double u[3] = {0., 0., 0.};
     AccF64 acc;
-    walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, lane, st, u, acc, wc);
+    walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, st, u, acc, wc);
     if (kMode == QV_FP64) { uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2]; }
     else acc.get(uc[0], uc[1], uc[2]);
   }
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   AccF64 acc;
   for (int sh = 1; sh < S.n; ++sh) {
     const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, lane, st, u, acc, wc);
+    walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, st, u, acc, wc);
   }
   // the output pointers are read from the kernel-argument segment here, after
   // the walk, so that they do not occupy scalar registers during it
