@@ -187,8 +187,9 @@ struct SrcChunk {
 // The register budget is that of 4 waves per SIMD for every tile shape;
 // the 512-source chunks use enough LDS (18-25 KB per one-wave workgroup) to
 // allow only 2-3, and the compiler reports that it missed the target.
-// Budgeting those shapes for their real occupancy measured slower (more
-// registers, longer schedules), so they keep the 4-wave budget.
+// Budgeting those shapes for their real occupancy made the fastest shape
+// (2 targets per lane, 512 sources) 4% slower, so they keep the 4-wave
+// budget.
 template <int kPer, int kSrc, bool kF64>
 __global__ void __launch_bounds__(64, 4) k_batch_step(BatchArgs A, Shells S) {
   __shared__ SrcChunk<kSrc, kF64> t;
