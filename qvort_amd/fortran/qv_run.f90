@@ -230,8 +230,16 @@ contains
     end do
     close(fh)
     if (.not. box_size > 0._c_double) call stop_with('init_setup:', 'box size is less than zero')
-    if (velocity == 'Tree' .and. .not. tree_theta > 0._c_double) &
-      call stop_with('init_setup', 'tree_theta must be > 0 for the Tree velocity')
+    ! init_setup's checks of the velocity choice (initial.f90:355-363)
+    if (velocity == 'Tree') then
+      if (tree_theta < epsilon(0._c_double)) call stop_with('init.mod:init_setup', &
+        'runnning with tree velocity but tree_theta is zero')
+      if (tree_theta > 0.6_c_double) then
+        write (*,*) '-------------------------WARNING----------------------------'
+        write (*,*) 'init_setup', ": ", 'tree_theta>0.6, the vortices may be unstable'
+        write (*,*) '------------------------------------------------------------'
+      end if
+    end if
   end subroutine
 
   subroutine init_filament()
