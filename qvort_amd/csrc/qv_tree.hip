@@ -587,8 +587,8 @@ int tree_build(Ctx &c) {
   QV_TRY(hipStreamSynchronize(st));
   if (c.h_counters[1] & kFlagDepth) return QV_ERR_TREE_DEPTH;
   const int64_t nnodes = c.h_counters[0];
-  // the walk addresses node records with 32-bit byte offsets (64 B each)
-  if (nnodes >= ((int64_t)1 << 26)) return QV_ERR_ARG;
+  // node indices are 32-bit throughout (the count itself is an int32 sum)
+  if (nnodes < 1 || nnodes >= (int64_t)INT32_MAX) return QV_ERR_ARG;
   QV_TRY(grow_tree(c.tree, nnodes));
   c.tree.nnodes = nnodes;
   // 6. node records

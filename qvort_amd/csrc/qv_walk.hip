@@ -18,13 +18,14 @@
 // d2 = (dx^2 + dy^2) + dz^2 (the reference's operand of sqrt) is >= the
 // node's threshold T, the smallest d2 for which the reference's test
 // w/(d-g) < theta holds (precomputed exactly at build time); nodes with
-// T < 0 take the full test.  The decisions are bit-identical to the
-// reference's.
+// T < 0 take the full test.  Given a node's centroid the decisions are
+// bit-identical to the reference's (the centroid is summed in key order
+// here, in slot order by the reference, and may differ in its last bits).
 //
-// Per node step the record arrives with one s_load_dwordx16 (32-bit byte
-// offset), the lane's bit of the walk mask is taken with inverse_ballot (no
-// VALU), T < 0 is read from the sign of T's high word (a scalar test), and
-// the accepting lanes are M & ballot(d2 >= T).  In FP32C every internal
+// Per node step the record arrives with one s_load_dwordx16, the lane's
+// bit of the walk mask is taken with inverse_ballot (no VALU), T < 0 is
+// read from the sign of T's high word (a scalar test), and the accepting
+// lanes are M & ballot(d2 >= T).  In FP32C every internal
 // node's multipole is evaluated with weight 0 on the lanes that do not
 // accept: a uniform branch around it saves nothing measurable on the 7% of
 // steps in which no lane accepts.  Each step's term (leaf or cell) is added
@@ -241,9 +242,9 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
     s_[0] = __shfl(sv[0], 0); s_[1] = __shfl(sv[1], 0); s_[2] = __shfl(sv[2], 0);
   }
   while (true) {
-    // byte offset of record n in 32 bits (64 B per record, 16 floats of
-    // mp32 per node: the same offset); nnodes < 2^26
-    const uint32_t off = (uint32_t)n << 6;
+    // byte offset of record n (64 B per record, 16 floats of mp32 per node:
+    // the same offset), 64-bit: trees beyond 2^26 nodes
+    const uint64_t off = (uint64_t)(uint32_t)n << 6;
     const u32x16 raw = *reinterpret_cast<const u32x16 *>(recb + off);
     NodeRec rec;
     __builtin_memcpy(&rec, &raw, sizeof(NodeRec));
