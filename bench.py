@@ -186,6 +186,41 @@ def copy_calibration(gib: float = 2.0, reps: int = 5) -> None:
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}))
 
 
+def run_next_rows(fil, theta: float) -> dict:
+    """The calls next to the velocity stage on the same workload, wall-clock
+    per call from Python (host-device copies included): the reconnection
+    step's neighbour search on the stage's tree (pclose_tree), the velocity
+    at 100,000 random probe points by tree_walk_general's rules
+    (mesh_velocity), and one topology step (pinsert on the host, the search,
+    precon and premove on the host, state down and back up)."""
+    from qvort_amd import gpu, line
+    p = gpu.params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
+                   delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6, precision="fp32c")
+    y = np.random.Generator(np.random.PCG64(7)).uniform(-0.1, 0.1, size=(100_000, 3))
+    out = {}
+    with gpu.Context(p) as c:
+        c.upload_filaments(fil)
+        c.velocity_only()
+        c.closest()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            c.closest()
+        out["closest_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
+        c.probe_velocity(y[:1000])
+        t0 = time.perf_counter()
+        c.probe_velocity(y)
+        out["probe_tree_100k_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        st = line.LineState.from_arrays(capacity=2 * fil.pcount + 64, x=fil.x,
+                                        infront=fil.infront, behind=fil.behind)
+        cnt = line.new_counts()
+        c.ab_step()
+        t0 = time.perf_counter()
+        line.topology_step(c, p, st, cnt, search=True)
+        out["topology_step_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        out["topology_counts"] = line.counts_dict(cnt)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +232,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-next-rows", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32c"], default=None,
                     help="profile one precision (no JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -275,6 +311,9 @@ def main():
         out["ensemble"] = run_ensemble(args.ensemble_steps)
         print(f"ensemble: {out['ensemble']['ms_per_step']:.3f} ms/step", file=sys.stderr,
               flush=True)
+    if not args.no_next_rows:
+        out["next_rows"] = run_next_rows(fil, args.theta)
+        print(f"next rows: {out['next_rows']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fil, args.theta, args.cpu_sample)
     print(json.dumps(out), flush=True)

@@ -1,16 +1,23 @@
 // qv_closest.hip -- neighbour search of the reconnection step.
 //
-// Tree mode: pclose_tree / closest_tree (tree.f90:219-278).  One thread per
-// slot walks the octree of the last velocity stage (built, like
-// construct_tree's, from the positions before the step's motion) with the
-// slot's current position, in DFS preorder without a stack: a taken node
-// continues at its skip pointer, an opened node at the next record (its
-// first stored child; empty children are not stored and would return).  A
-// node is taken when it holds one point or width/dist < 0.5; when dist <
-// closestd, a many-point node sets closest = 0 (closestd unchanged) and a
-// one-point node offers j = behind(infront_at_build(point)) with the current
-// behind, taken unless j is i, infront(i) or behind(i).  closestd starts at
-// 100 for every slot, empty or not (tree.f90:227).
+// Tree mode: pclose_tree / closest_tree (tree.f90:219-278).  Each slot
+// walks the octree of the last velocity stage (built, like construct_tree's,
+// from the positions before the step's motion) with the slot's current
+// position, in DFS preorder: a taken node is left for its skip pointer, an
+// opened node continues at the next record (its first stored child; empty
+// children are not stored and would return).  A node is taken when it
+// holds one point or width/dist < 0.5; when dist < closestd, a many-point
+// node sets closest = 0 (closestd unchanged) and a one-point node offers
+// j = behind(infront_at_build(point)) with the current behind, taken unless
+// j is i, infront(i) or behind(i).  closestd starts at 100 for every slot,
+// empty or not (tree.f90:227).
+//
+// The walk is wavefront-cooperative, as the velocity walk is: the 64 slots
+// of a wavefront (consecutive slots, mostly consecutive points of a loop)
+// share one DFS whose current node is read with scalar loads; each lane
+// takes part only in the nodes of its own walk (its bit of the mask), so it
+// meets them in the same order and updates closest / closestd exactly as
+// its own recursive walk would.
 //
 // Direct mode: pclose (line.f90:205-234), all pairs in slot order through
 // LDS tiles; empty slots keep closest and closestd.
@@ -39,43 +46,85 @@ struct CloseArgs {
   double *cd;
 };
 
+struct alignas(16) CloseEntry {
+  int32_t node, end;
+  uint64_t mask;
+};
+
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t uni64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __global__ void __launch_bounds__(kBlock) k_closest_tree(CloseArgs A) {
+  __shared__ CloseEntry stack[kBlock / kWave][kStackDepth];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
-  A.cd[i] = 100.;
-  const int32_t fi = A.inf[i];
-  if (fi == 0) return;
-  const int32_t bi = A.beh[i];
-  const double x0 = A.X[i], x1 = A.Y[i], x2 = A.Z[i];
-  int32_t best = A.cl[i];
+  const bool valid = i < A.n;
+  if (valid) A.cd[i] = 100.;
+  const bool act = valid && A.inf[i] != 0;
+  int32_t fi = 0, bi = 0, best = 0;
+  double x0 = 0., x1 = 0., x2 = 0.;
+  if (act) {
+    fi = A.inf[i];
+    bi = A.beh[i];
+    best = A.cl[i];
+    x0 = A.X[i]; x1 = A.Y[i]; x2 = A.Z[i];
+  }
   double bd = 100.;
-  int64_t n = 0;
-  while (n < A.nnodes) {
+  uint64_t M = __ballot(act);
+  if (M == 0) return;  // the whole wavefront: M is the same in every lane
+  CloseEntry *st = stack[threadIdx.x / kWave];
+  int n = 0, e = (int)A.nnodes, sp = 0;
+  while (true) {
     const NodeRec &r = A.rec[n];
-    const double d0 = x0 - r.cx, d1 = x1 - r.cy, d2 = x2 - r.cz;
+    const double cx = r.cx, cy = r.cy, cz = r.cz;
+    const int skip = uni(r.skip);
+    const int32_t cnt = uni(A.count[n]);
+    const double w = A.geo[n].x;
+    const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
+    const double d0 = x0 - cx, d1 = x1 - cy, d2 = x2 - cz;
     const double dist = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-    const int32_t cnt = A.count[n];
-    const double theta = A.geo[n].x / dist;
-    if (cnt == 1 || theta < .5) {
-      if (dist < bd) {
-        if (cnt > 1) {
-          best = 0;
-        } else {
-          const int32_t q = A.linf[n];
-          const int32_t j = q >= 0 ? A.beh[q] : 0;  // q < 0 only for a 1-slot root
-          if (j != i + 1 && fi != j && bi != j) {
-            best = j;
-            bd = dist;
-          }
+    const bool taken = mine && (cnt == 1 || w / dist < .5);
+    if (taken && dist < bd) {
+      if (cnt > 1) {
+        best = 0;
+      } else {
+        const int32_t q = A.linf[n];
+        const int32_t j = q >= 0 ? A.beh[q] : 0;  // q < 0 only for a 1-slot root
+        if (j != i + 1 && fi != j && bi != j) {
+          best = j;
+          bd = dist;
         }
       }
-      n = r.skip;
-    } else {
+    }
+    const uint64_t O = M & ~__ballot(taken);
+    if (O != 0 && n + 1 != skip) {
+      // descend into the children [n+1, skip) with the opening lanes
+      if (skip != e) {
+        st[sp].node = skip;
+        st[sp].end = e;
+        st[sp].mask = M;
+        ++sp;
+      }
+      e = skip;
       n = n + 1;
+      M = O;
+    } else if (skip != e) {
+      n = skip;  // next sibling, same lanes
+    } else {
+      if (sp == 0) break;
+      --sp;
+      n = uni(st[sp].node);
+      e = uni(st[sp].end);
+      M = uni64(st[sp].mask);
     }
   }
-  A.cl[i] = best;
-  A.cd[i] = bd;
+  if (act) {
+    A.cl[i] = best;
+    A.cd[i] = bd;
+  }
 }
 
 constexpr int kTile = 256;

@@ -250,17 +250,25 @@ class Context:
     def step(self, nsteps: int) -> None:
         _check(lib().qv_step(self._h, nsteps), "qv_step")
 
-    def download(self):
-        """(x, u, u1, u2, u3)."""
+    def _outs(self, count: int, out):
         n = self.pcount
-        arrs = [np.zeros((n, 3)) for _ in range(5)]
+        if out is None:
+            return [np.zeros((n, 3)) for _ in range(count)]
+        if len(out) != count or any(a.shape != (n, 3) or a.dtype != np.float64
+                                    or not a.flags.c_contiguous for a in out):
+            raise ValueError(f"out must be {count} C-contiguous float64 ({n},3) arrays")
+        return list(out)
+
+    def download(self, out=None):
+        """(x, u, u1, u2, u3); out: five (pcount,3) float64 arrays to fill."""
+        arrs = self._outs(5, out)
         _check(lib().qv_download(self._h, *[_dp(a) for a in arrs]), "qv_download")
         return tuple(arrs)
 
-    def download_ghosts(self):
-        """(ghosti, ghostb, ghostii, ghostbb) of the last ghostp."""
-        n = self.pcount
-        arrs = [np.zeros((n, 3)) for _ in range(4)]
+    def download_ghosts(self, out=None):
+        """(ghosti, ghostb, ghostii, ghostbb) of the last ghostp; out: four
+        (pcount,3) float64 arrays to fill."""
+        arrs = self._outs(4, out)
         _check(lib().qv_download_ghosts(self._h, *[_dp(a) for a in arrs]), "qv_download_ghosts")
         return tuple(arrs)
 

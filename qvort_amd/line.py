@@ -98,12 +98,13 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
     holds the current links (and closest/closestd) between calls; the device
     sends back positions, velocities and ghosts only."""
     n = ctx.pcount
-    x, u, u1, u2, u3 = ctx.download()
-    gi, gb, gii, gbb = ctx.download_ghosts()
+    if n > st.capacity:
+        raise ValueError("the line state holds fewer slots than the device")
     st.pcount = n
-    for name, a in (("x", x), ("u", u), ("u1", u1), ("u2", u2), ("u3", u3), ("ghosti", gi),
-                    ("ghostb", gb), ("ghostii", gii), ("ghostbb", gbb)):
-        getattr(st, "_" + name)[:n] = a
+    # straight into the state's arrays (their first n rows are contiguous)
+    ctx.download(out=[getattr(st, "_" + k)[:n] for k in ("x", "u", "u1", "u2", "u3")])
+    ctx.download_ghosts(out=[getattr(st, "_" + k)[:n]
+                             for k in ("ghosti", "ghostb", "ghostii", "ghostbb")])
     pinsert(p, st, counts)
     if search:
         ctx.upload(st.x, st.infront, st.behind)
