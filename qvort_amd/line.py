@@ -19,23 +19,53 @@ VEC_FIELDS = ("x", "u", "u1", "u2", "u3", "ghosti", "ghostb", "ghostii", "ghostb
 INT_FIELDS = ("infront", "behind", "closest")
 
 
-class LineState:
-    """Host arrays of `capacity` slots, of which the first `pcount` are f(:)."""
+def _pinned_allocator():
+    """numpy arrays in page-locked host memory (torch's pinned allocator) when
+    a GPU is visible, so that the state's copies to and from the device run
+    at DMA speed; None otherwise."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if not torch.cuda.is_available():
+        return None
+    keep = []
 
-    def __init__(self, pcount: int, capacity: int | None = None):
+    def alloc(shape, dtype):
+        t = torch.zeros(shape, dtype={np.float64: torch.float64, np.int32: torch.int32}[dtype],
+                        pin_memory=True)
+        keep.append(t)
+        return t.numpy()
+    alloc.keep = keep
+    return alloc
+
+
+class LineState:
+    """Host arrays of `capacity` slots, of which the first `pcount` are f(:);
+    page-locked when a GPU is visible (pinned=None) or asked for."""
+
+    def __init__(self, pcount: int, capacity: int | None = None, pinned: bool | None = None):
         cap = max(pcount, capacity if capacity is not None else pcount + pcount // 8 + 64)
         self.pcount = int(pcount)
         self.capacity = int(cap)
+        alloc = _pinned_allocator() if pinned in (None, True) else None
+        if pinned and alloc is None:
+            raise RuntimeError("pinned host memory needs torch with a visible GPU")
+        if alloc is None:
+            def alloc(shape, dtype):
+                return np.zeros(shape, dtype=dtype)
+        self._alloc = alloc  # keeps the pinned buffers alive
         for n in VEC_FIELDS:
-            setattr(self, "_" + n, np.zeros((cap, 3)))
+            setattr(self, "_" + n, alloc((cap, 3), np.float64))
         for n in INT_FIELDS:
-            setattr(self, "_" + n, np.zeros(cap, dtype=np.int32))
-        self._closestd = np.zeros(cap)
+            setattr(self, "_" + n, alloc((cap,), np.int32))
+        self._closestd = alloc((cap,), np.float64)
 
     @classmethod
-    def from_arrays(cls, capacity: int | None = None, **arrays) -> "LineState":
+    def from_arrays(cls, capacity: int | None = None, pinned: bool | None = None,
+                    **arrays) -> "LineState":
         n = np.asarray(arrays["x"]).shape[0]
-        st = cls(n, capacity)
+        st = cls(n, capacity, pinned)
         for name, a in arrays.items():
             if a is not None:
                 getattr(st, "_" + name)[:n] = a
