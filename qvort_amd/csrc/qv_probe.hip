@@ -199,19 +199,18 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S) {
         const double d3 = (dist * dist) * dist;
         const double d5 = (((dist * dist) * dist) * dist) * dist;
         for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
+        // the reference's k, l, m loops (tree.f90:466-478) with the terms of
+        // a zero Levi-Civita symbol left out: each adds (0 * finite) / d = 0
+        // to its sum, which leaves the sum as it is, so the remaining terms
+        // meet every sum in the reference's order
         for (int k = 0; k < 3; ++k) {
           for (int l = 0; l < 3; ++l) {
+            if (k == l) continue;
+            const int m = 3 - k - l;
+            const double e = ((m + 1) % 3 == k) ? 1. : -1.;
             const double dcl = (v[0] * C[l] + v[1] * C[3 + l]) + v[2] * C[6 + l];
-            for (int m = 0; m < 3; ++m) {
-              double e = 0.;
-              if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
-              ub[m] = ub[m] - (e * C[3 * k + l]) / d3;
-            }
-            for (int m = 0; m < 3; ++m) {
-              double e = 0.;
-              if (!(m == k || k == l || m == l)) e = (((m + 1) % 3 == k) ? 1. : -1.);
-              ub[m] = ub[m] + (((3. * e) * v[k]) * dcl) / d5;
-            }
+            ub[m] = ub[m] - (e * C[3 * k + l]) / d3;
+            ub[m] = ub[m] + (((3. * e) * v[k]) * dcl) / d5;
           }
         }
         for (int k = 0; k < 3; ++k)
