@@ -520,13 +520,13 @@ int qv_probe_velocity(qv_ctx *ctx, int32_t nprobe, const double *y, double *u, d
   if (nprobe == 0) return QV_OK;
   if (c->probe_cap < nprobe) {
     TRY(hipStreamSynchronize(c->stream));
-    TRY(dalloc(c->probe_buf, (size_t)15 * nprobe));
+    TRY(dalloc(c->probe_buf, (size_t)18 * nprobe));
     c->probe_cap = nprobe;
   }
   double *d_y = c->probe_buf, *d_u = d_y + 3 * (size_t)nprobe, *d_g = d_u + 3 * (size_t)nprobe;
   TRY(hipMemcpyAsync(d_y, y, 3 * (size_t)nprobe * sizeof(double), hipMemcpyHostToDevice,
                      c->stream));
-  TRY(launch_probe(*c, nprobe, d_y, d_u, grad ? d_g : nullptr, tree,
+  TRY(launch_probe(*c, nprobe, d_y, d_u, grad ? d_g : nullptr, d_g + 9 * (size_t)nprobe, tree,
                    (flags & QV_PROBE_REFERENCE_SCALE) != 0,
                    (flags & QV_PROBE_EMPTY_CORE) != 0));
   TRY(hipMemcpyAsync(u, d_u, 3 * (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost,

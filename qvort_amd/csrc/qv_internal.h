@@ -90,7 +90,7 @@ struct Ctx {
   // ghosts of the last ghostp (velocity stage or a probe call with
   // QV_PROBE_GHOSTP); an upload invalidates them
   bool ghosts_valid = false;
-  double *probe_buf = nullptr;  // probes (3n), u (3n), gradient (9n)
+  double *probe_buf = nullptr;  // probes (3n), u (3n), gradient (9n), sort keys (3n)
   int32_t probe_cap = 0;
   // state (SoA), slot order
   Vec3 x, gi, gb, gii, gbb, u, uli, ubs, u1, u2, u3;
@@ -171,6 +171,7 @@ hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 hipError_t launch_closest(Ctx &c, bool tree);
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
+                        double *d_work,
                         bool tree, bool ref_scale, bool empty_core);
 
 }  // namespace qv

@@ -12,12 +12,18 @@
 //                  replaces the six extra evaluations and central
 //                  differences of velocity_quasip (hamiltonian.f90:64-79).
 //   velocity Tree: tree_walk_general (tree.f90:425-488) over the tree of the
-//                  last velocity stage, one thread per probe, stackless DFS
-//                  on skip pointers.  The reference adds leaves as
+//                  last velocity stage.  The probes are put in the order of
+//                  their 63-bit Morton keys in the box (a radix sort), and
+//                  the 64 probes of a wavefront share one DFS per shell,
+//                  node records by scalar loads, each lane taking part only
+//                  in its own walk's nodes, in the same order as its own
+//                  recursive walk.  The reference adds leaves as
 //                  u_bs/4/pi (no Gamma) and cells without Gamma/(4 pi); by
 //                  default both get Gamma/(4 pi) here, and
 //                  QV_PROBE_REFERENCE_SCALE keeps the reference's factors.
 #include "qv_internal.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #pragma clang fp contract(off)
 
@@ -151,83 +157,156 @@ __global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
     for (int k = 0; k < 9; ++k) A.g[9 * q + k] = G[k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S) {
+// 21 bits per axis of the position in the box (clamped), interleaved
+__device__ inline uint64_t spread21(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_probe_keys(int nprobe, const double *py, double L,
+                                                       uint64_t *key, int32_t *idx) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= A.nprobe) return;
-  const double y[3] = {A.py[3 * q], A.py[3 * q + 1], A.py[3 * q + 2]};
+  if (q >= nprobe) return;
+  uint64_t k = 0;
+  for (int d = 0; d < 3; ++d) {
+    double f = (py[3 * q + d] + L / 2.) / L;
+    f = fmin(fmax(f, 0.), 1. - 1e-12);
+    k |= spread21((uint64_t)(f * 2097152.)) << d;
+  }
+  key[q] = k;
+  idx[q] = q;
+}
+
+struct alignas(16) ProbeEntry {
+  int32_t node, end;
+  uint64_t mask;
+};
+
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t uni64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// lane q of the sorted order works on probe order[q]
+__global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
+                                                       const int32_t *__restrict__ order) {
+  __shared__ ProbeEntry stack[kBlock / kWave][kStackDepth];
+  const int qs = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = qs < A.nprobe;
+  const int q = act ? order[qs] : 0;
+  double y[3] = {0., 0., 0.};
+  if (act)
+    for (int k = 0; k < 3; ++k) y[k] = A.py[3 * q + k];
+  const uint64_t all = __ballot(act);
+  if (all == 0) return;  // the whole wavefront
+  ProbeEntry *st = stack[threadIdx.x / kWave];
   double u[3] = {0., 0., 0.};
   for (int sh = 0; sh < S.n; ++sh) {
     const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    int64_t n = 0;
-    while (n < A.nnodes) {
+    int n = 0, e = (int)A.nnodes, sp = 0;
+    uint64_t M = all;
+    while (true) {
       const NodeRec &r = A.rec[n];
       const double c3[3] = {r.cx, r.cy, r.cz};
+      const int skip = uni(r.skip);
+      const int32_t cnt = uni(A.count[n]);
+      const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
       const double d0 = y[0] - (c3[0] + s[0]), d1 = y[1] - (c3[1] + s[1]),
                    d2 = y[2] - (c3[2] + s[2]);
       const double dist = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-      const int32_t cnt = A.count[n];
-      if (dist < 2.220446049250313e-16) {  // epsilon(0.): avoid 1/0 (tree.f90:441)
-        n = r.skip;
-        continue;
-      }
-      const double theta = A.geo[n].x / dist;
+      // epsilon(0.): the node is left alone (avoids 1/0, tree.f90:441)
+      const bool live = mine && !(dist < 2.220446049250313e-16);
+      bool taken;
       if (cnt == 1) {
-        // circ of a one-point cell is (0 + ghosti) - x (tree.f90:160-165)
-        double R1[3], R2[3];
-        for (int k = 0; k < 3; ++k) {
-          const double circ = r.g[k] - c3[k];
-          R1[k] = (c3[k] + s[k]) - y[k];
-          R2[k] = ((c3[k] + circ) + s[k]) - y[k];
-        }
-        const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
-        const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
-        const double cc = (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
-        const double f = (a + b) / cc;
-        const double ub[3] = {f * (R1[1] * R2[2] - R1[2] * R2[1]),
-                              f * (R1[2] * R2[0] - R1[0] * R2[2]),
-                              f * (R1[0] * R2[1] - R1[1] * R2[0])};
-        for (int k = 0; k < 3; ++k)
-          u[k] = u[k] + (A.ref_scale ? (ub[k] / 4.) / kPi : ((ub[k] * A.qc) / 4.) / kPi);
-        n = r.skip;
-      } else if (theta < A.theta) {
-        const double *mp = A.mp + 12 * n;
-        const double *circ = mp;
-        const double *C = mp + 3;  // C[3*i + k] = Cij(i+1, k+1)
-        const double v[3] = {d0, d1, d2};
-        double ub[3] = {circ[1] * v[2] - circ[2] * v[1], circ[2] * v[0] - circ[0] * v[2],
-                        circ[0] * v[1] - circ[1] * v[0]};
-        const double d3 = (dist * dist) * dist;
-        const double d5 = (((dist * dist) * dist) * dist) * dist;
-        for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
-        // the reference's k, l, m loops (tree.f90:466-478) with the terms of
-        // a zero Levi-Civita symbol left out: each adds (0 * finite) / d = 0
-        // to its sum, which leaves the sum as it is, so the remaining terms
-        // meet every sum in the reference's order
-        for (int k = 0; k < 3; ++k) {
-          for (int l = 0; l < 3; ++l) {
-            if (k == l) continue;
-            const int m = 3 - k - l;
-            const double e = ((m + 1) % 3 == k) ? 1. : -1.;
-            const double dcl = (v[0] * C[l] + v[1] * C[3 + l]) + v[2] * C[6 + l];
-            ub[m] = ub[m] - (e * C[3 * k + l]) / d3;
-            ub[m] = ub[m] + (((3. * e) * v[k]) * dcl) / d5;
+        taken = mine;
+        if (live) {
+          // circ of a one-point cell is (0 + ghosti) - x (tree.f90:160-165)
+          double R1[3], R2[3];
+          for (int k = 0; k < 3; ++k) {
+            const double circ = r.g[k] - c3[k];
+            R1[k] = (c3[k] + s[k]) - y[k];
+            R2[k] = ((c3[k] + circ) + s[k]) - y[k];
           }
+          const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
+          const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
+          const double cc =
+              (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
+          const double f = (a + b) / cc;
+          const double ub[3] = {f * (R1[1] * R2[2] - R1[2] * R2[1]),
+                                f * (R1[2] * R2[0] - R1[0] * R2[2]),
+                                f * (R1[0] * R2[1] - R1[1] * R2[0])};
+          for (int k = 0; k < 3; ++k)
+            u[k] = u[k] + (A.ref_scale ? (ub[k] / 4.) / kPi : ((ub[k] * A.qc) / 4.) / kPi);
         }
-        for (int k = 0; k < 3; ++k)
-          u[k] = A.ref_scale ? u[k] + ub[k] : u[k] + ((ub[k] * A.qc) / 4.) / kPi;
-        n = r.skip;
       } else {
+        const double theta = A.geo[n].x / dist;
+        const bool cell = live && theta < A.theta;
+        taken = mine && (!live || cell);
+        if (cell) {
+          const double *mp = A.mp + 12 * (int64_t)n;
+          const double *circ = mp;
+          const double *C = mp + 3;  // C[3*i + k] = Cij(i+1, k+1)
+          const double v[3] = {d0, d1, d2};
+          double ub[3] = {circ[1] * v[2] - circ[2] * v[1], circ[2] * v[0] - circ[0] * v[2],
+                          circ[0] * v[1] - circ[1] * v[0]};
+          const double d3 = (dist * dist) * dist;
+          const double d5 = (((dist * dist) * dist) * dist) * dist;
+          for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
+          // the reference's k, l, m loops (tree.f90:466-478) with the terms
+          // of a zero Levi-Civita symbol left out: each adds (0 * finite) / d
+          // = 0 to its sum, which leaves the sum as it is, so the remaining
+          // terms meet every sum in the reference's order
+          for (int k = 0; k < 3; ++k) {
+            for (int l = 0; l < 3; ++l) {
+              if (k == l) continue;
+              const int m = 3 - k - l;
+              const double ee = ((m + 1) % 3 == k) ? 1. : -1.;
+              const double dcl = (v[0] * C[l] + v[1] * C[3 + l]) + v[2] * C[6 + l];
+              ub[m] = ub[m] - (ee * C[3 * k + l]) / d3;
+              ub[m] = ub[m] + (((3. * ee) * v[k]) * dcl) / d5;
+            }
+          }
+          for (int k = 0; k < 3; ++k)
+            u[k] = A.ref_scale ? u[k] + ub[k] : u[k] + ((ub[k] * A.qc) / 4.) / kPi;
+        }
+      }
+      const uint64_t O = M & ~__ballot(taken);
+      if (O != 0 && n + 1 != skip) {
+        if (skip != e) {
+          st[sp].node = skip;
+          st[sp].end = e;
+          st[sp].mask = M;
+          ++sp;
+        }
+        e = skip;
         n = n + 1;
+        M = O;
+      } else if (skip != e) {
+        n = skip;
+      } else {
+        if (sp == 0) break;
+        --sp;
+        n = uni(st[sp].node);
+        e = uni(st[sp].end);
+        M = uni64(st[sp].mask);
       }
     }
   }
-  for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
+  if (act)
+    for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
 }
 
 }  // namespace
 
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
-                        bool tree, bool ref_scale, bool empty_core) {
+                        double *d_work, bool tree, bool ref_scale, bool empty_core) {
   ProbeArgs A;
   A.n = c.pcount;
   A.nprobe = nprobe;
@@ -249,8 +328,32 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
   A.mp = c.tree.mp;
   const Shells S = make_shells(c.p);
   const dim3 g((nprobe + kBlock - 1) / kBlock);
-  if (tree)
-    hipLaunchKernelGGL(k_probe_tree, g, dim3(kBlock), 0, c.stream, A, S);
+  if (tree) {
+    // probes in Morton order: 2 x nprobe keys, then 2 x nprobe indices
+    uint64_t *key = reinterpret_cast<uint64_t *>(d_work), *key_s = key + nprobe;
+    int32_t *idx = reinterpret_cast<int32_t *>(key_s + nprobe), *idx_s = idx + nprobe;
+    hipLaunchKernelGGL(k_probe_keys, g, dim3(kBlock), 0, c.stream, nprobe, d_y, c.p.box_size,
+                       key, idx);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    size_t need = 0;
+    err = rocprim::radix_sort_pairs(nullptr, need, key, key_s, idx, idx_s, nprobe, 0, 63,
+                                    c.stream);
+    if (err != hipSuccess) return err;
+    if (need > c.sort_tmp_bytes) {
+      if ((err = hipStreamSynchronize(c.stream)) != hipSuccess) return err;
+      (void)hipFree(c.sort_tmp);
+      c.sort_tmp = nullptr;
+      c.sort_tmp_bytes = 0;
+      if ((err = hipMalloc(&c.sort_tmp, need)) != hipSuccess) return err;
+      c.sort_tmp_bytes = need;
+    }
+    size_t have = c.sort_tmp_bytes;
+    err = rocprim::radix_sort_pairs(c.sort_tmp, have, key, key_s, idx, idx_s, nprobe, 0, 63,
+                                    c.stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_probe_tree, g, dim3(kBlock), 0, c.stream, A, S, idx_s);
+  }
   else if (d_g)
     hipLaunchKernelGGL(k_probe_bs<true>, g, dim3(kBlock), 0, c.stream, A, S);
   else
