@@ -56,9 +56,12 @@ hipError_t vzero(Vec3 &v, size_t n, hipStream_t s) {
     if (e_ != hipSuccess) return QV_ERR_HIP;   \
   } while (0)
 
+// slot buffers grow by a quarter at a time: pinsert's appended slots make
+// pcount creep up from one upload to the next
 int reserve(Ctx &c, int32_t n) {
   if (n <= c.cap) return QV_OK;
-  const size_t cap = (size_t)n;
+  const size_t cap = (size_t)n + (size_t)n / 4 + 1024;
+  if (cap > (size_t)INT32_MAX) return QV_ERR_ARG;
   Vec3 *vs[] = {&c.x, &c.gi, &c.gb, &c.gii, &c.gbb, &c.u, &c.uli, &c.ubs,
                 &c.u1, &c.u2, &c.u3, &c.xs, &c.gs};
   for (Vec3 *v : vs) TRY(valloc(*v, cap));
@@ -72,7 +75,7 @@ int reserve(Ctx &c, int32_t n) {
   TRY(dalloc(c.slot_s, cap));
   TRY(dalloc(c.nnode_of, cap + 1));
   TRY(dalloc(c.node_off, cap + 1));
-  c.cap = n;
+  c.cap = (int32_t)cap;
   return QV_OK;
 }
 
@@ -486,9 +489,10 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
   if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
   if (c->close_cap < c->pcount) {
     TRY(hipStreamSynchronize(c->stream));
-    TRY(dalloc(c->close_i, c->pcount));
-    TRY(dalloc(c->close_d, c->pcount));
-    c->close_cap = c->pcount;
+    const int32_t cap = c->cap;  // the slot buffers' capacity (>= pcount)
+    TRY(dalloc(c->close_i, cap));
+    TRY(dalloc(c->close_d, cap));
+    c->close_cap = cap;
   }
   const size_t n = c->pcount;
   TRY(hipMemcpyAsync(c->close_i, closest, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
