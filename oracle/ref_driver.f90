@@ -26,6 +26,14 @@
 !>                   own type(qvort) under this compiler.
 !>   construct_tree's leaf print data/tree_mesh%03d.log (tree.f90:116-123)
 !>                   when run.in sets tree_print T and shots divides itime
+!>   mesh_velocity   timestep.f90:374-452, when run.in sets mesh_size > 0: the
+!>                   grid of setup_mesh (initial.f90:541-574, restated: the
+!>                   module does not build here), mesh_velocity after the
+!>                   motion when mesh_shots divides itime and itime >
+!>                   mesh_print_delay (run.f90:56-58), and data/mesh%03d.dat
+!>                   with print_mesh's writes (output.f90:305-321, restated)
+!>                   numbered as mesh_shot_print_calls does (output.f90:32-41)
+!>                   when shots and mesh_shots divide itime (run.f90:111-124)
 !>   the line operations of run.f90:60-72 (flags bit 8): pinsert
 !>                   (line.f90:18-108), and when recon_shots divides itime
 !>                   pclose_tree / pclose, precon (line.f90:238-252, run.in
@@ -87,6 +95,7 @@ program qv_ref_driver
 
   call read_run_file()
   call set_boundary_flags()
+  if (mesh_size > 0) call setup_mesh_grid()
 
   open(unit=41, file='state.bin', access='stream', form='unformatted', status='old', action='read')
   read(41) version, pcount, nsteps_run, flags, nsample
@@ -168,6 +177,9 @@ program qv_ref_driver
     else
       call pmotion
     end if
+    if (mesh_size > 0) then
+      if ((mod(itime,mesh_shots)==0).and.(itime>mesh_print_delay)) call mesh_velocity
+    end if
     if (itime == nstart) then
       do i = 1, pcount
         u_first(:,i) = f(i)%u
@@ -209,6 +221,9 @@ program qv_ref_driver
         call enforce_periodic_z
     end select
     t = t + dt
+    if (mesh_size > 0 .and. mod(itime, shots) == 0 .and. mod(itime, mesh_shots) == 0) then
+      if (itime > mesh_print_delay) call write_mesh((itime-mesh_print_delay)/mesh_shots)
+    end if
     if (tree_theta > 0) then
       call empty_tree(vtree)
       deallocate(vtree%parray) ; deallocate(vtree)
@@ -280,6 +295,42 @@ contains
       case('openxy')
         periodic_bc_notxy = .true.
     end select
+  end subroutine
+
+  !> setup_mesh's grid (initial.f90:550-571): mesh(k,j,i)%x = (x_i, y_j, z_k)
+  subroutine setup_mesh_grid()
+    integer :: ii, jj, kk
+    real :: xx, yy, zz
+    mesh_delta=real(box_size)/mesh_size
+    allocate(mesh(mesh_size,mesh_size,mesh_size))
+    do kk=1, mesh_size
+      do jj=1, mesh_size
+        do ii=1, mesh_size
+          xx=mesh_delta*real(2*ii-1)/2.-(box_size/2.)
+          yy=mesh_delta*real(2*jj-1)/2.-(box_size/2.)
+          zz=mesh_delta*real(2*kk-1)/2.-(box_size/2.)
+          mesh(kk,jj,ii)%x(1)=xx ; mesh(kk,jj,ii)%x(2)=yy ; mesh(kk,jj,ii)%x(3)=zz
+          mesh(kk,jj,ii)%u_sup=0. ; mesh(kk,jj,ii)%u_norm=0.
+        end do
+      end do
+    end do
+  end subroutine
+
+  !> print_mesh's writes (output.f90:311-321)
+  subroutine write_mesh(filenumber)
+    integer, intent(in) :: filenumber
+    character(len=40) :: print_file
+    write(unit=print_file,fmt="(a,i3.3,a)")"./data/mesh",filenumber,".dat"
+    open(unit=92,file=print_file,form='unformatted',status='replace',access='stream')
+      write(92) t
+      write(92) mesh(mesh_size/2,mesh_size/2,1:mesh_size)%x(1)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_norm(1)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_norm(2)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_norm(3)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_sup(1)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_sup(2)
+      write(92) mesh(1:mesh_size,1:mesh_size,1:mesh_size)%u_sup(3)
+    close(92)
   end subroutine
 
   !> the f(:) fields the line operations read or write, to line.bin

@@ -25,7 +25,8 @@ def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_siz
                 delta=5e-4, dt=1e-6, quant_circ=9.97e-4, corea=8.244023e-9,
                 tree_extra_correction=True, deriv_order="second", nsteps=1,
                 runge_kutta=False, shots=1000000, tree_print=False,
-                recon_shots=1000000) -> str:
+                recon_shots=1000000, mesh_size=0, mesh_shots=100,
+                mesh_print_delay=0) -> str:
     lines = [f"runge_kutta {'T' if runge_kutta else 'F'}",
         f"nsteps {nsteps}", f"shots {shots}", f"recon_shots {recon_shots}", f"dt {dt!r}",
         f"tree_print {'T' if tree_print else 'F'}",
@@ -34,6 +35,8 @@ def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_siz
         f"tree_theta {tree_theta!r}",
         f"tree_extra_correction {'T' if tree_extra_correction else 'F'}",
         f"deriv_order {deriv_order}", "initf single_loop", "pcount 16",
+        f"mesh_size {mesh_size}", f"mesh_shots {mesh_shots}",
+        f"mesh_print_delay {mesh_print_delay}",
     ]
     return "\n".join(lines) + "\n"
 
@@ -60,7 +63,8 @@ def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None
     restart from with data_restore's statements (fil is then ignored; the run
     continues at the restored itime + 1).  With tree_print=True and shots
     dividing a step, the reference's data/tree_mesh%03d.log files come back
-    as {"tree_mesh": {number: text}}."""
+    as {"tree_mesh": {number: text}}; with mesh_size > 0 the data/mesh%03d.dat
+    files of the mesh velocity as {"mesh": {number: bytes}}."""
     if not available():
         raise FileNotFoundError(DRIVER)
     flags = ((1 if dump_tree else 0) | (2 if timing else 0) | (64 if dump_var else 0)
@@ -91,6 +95,13 @@ def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None
                     meshes[int(name[9:-4])] = fh.read()
         if meshes:
             out["tree_mesh"] = meshes
+        grids = {}
+        for name in sorted(os.listdir(os.path.join(wd, "data"))):
+            if name.startswith("mesh") and name.endswith(".dat"):
+                with open(os.path.join(wd, "data", name), "rb") as fh:
+                    grids[int(name[4:-4])] = fh.read()
+        if grids:
+            out["mesh"] = grids
         if dump_var:
             with open(os.path.join(wd, "data", "var.dat"), "rb") as fh:
                 out["var_dat"] = fh.read()

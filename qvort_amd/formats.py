@@ -14,6 +14,11 @@
   one list-directed record per cell that is not split (pcount <= 1), in the
   order the cells are created: posx, posx+width, posy, posy+width, posz,
   posz+width, pcount (the Fortran runtime may wrap a record over lines).
+* ``data/mesh%03d.dat`` — print_mesh's velocity grid (output.f90:305-321):
+  stream access, t, the x coordinates of the n grid planes, then u_norm(1),
+  u_norm(2), u_norm(3), u_sup(1), u_sup(2), u_sup(3), each over the n**3
+  points of mesh(k,j,i) in Fortran order (k fastest), where mesh(k,j,i) sits
+  at (x_i, y_j, z_k) (setup_mesh, initial.f90:550-571).
 
 Records longer than 2**31 - 1 bytes (f(:) of more than 4.6M points) need the
 compiler's subrecord convention and are refused.
@@ -205,3 +210,57 @@ def tree_mesh_rows(nodes: np.ndarray) -> np.ndarray:
     leaf = nodes[nodes[:, 1] <= 1]
     px, py, pz, w = leaf[:, 3], leaf[:, 4], leaf[:, 5], leaf[:, 6]
     return np.stack([px, px + w, py, py + w, pz, pz + w, leaf[:, 1]], axis=1)
+
+
+def mesh_points(mesh_size: int, box_size: float) -> np.ndarray:
+    """setup_mesh's grid (initial.f90:545-571) as (n**3, 3) points in the
+    order of print_mesh's arrays: point k + n*j + n*n*i (0-based) is
+    mesh(k+1,j+1,i+1) at (c_i, c_j, c_k), c_m = delta*(2m+1)/2 - L/2 with
+    delta = L/n, evaluated in that operation order."""
+    n = int(mesh_size)
+    if n <= 0 or n % 2:
+        raise ValueError("mesh size must be a multiple of 2")
+    L = float(box_size)
+    d = L / n
+    c = d * np.arange(1, 2 * n, 2, dtype=np.float64) / 2.0 - (L / 2.0)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    # flat index k + n*j + n*n*i: i slowest, k fastest
+    return np.stack([c[ii], c[jj], c[kk]], axis=-1).reshape(-1, 3)
+
+
+def _mesh_size_from_bytes(nbytes: int) -> int:
+    # 8 (t) + 8 n (plane coordinates) + 48 n**3 (six fields)
+    n = int(round(((nbytes - 8) / 48.0) ** (1.0 / 3.0)))
+    for m in (n - 1, n, n + 1):
+        if m > 0 and 8 + 8 * m + 48 * m ** 3 == nbytes:
+            return m
+    raise FormatError(f"{nbytes} bytes is no mesh file")
+
+
+def read_mesh_dat(path) -> dict:
+    """print_mesh's file: t, mesh_size n, x (n plane coordinates), and
+    u_norm, u_sup as (n**3, 3) arrays in the point order of mesh_points."""
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    n = _mesh_size_from_bytes(len(raw))
+    t = struct.unpack_from("<d", raw, 0)[0]
+    x = np.frombuffer(raw, dtype="<f8", count=n, offset=8).copy()
+    f = np.frombuffer(raw, dtype="<f8", offset=8 + 8 * n).reshape(6, n ** 3)
+    return {"t": t, "mesh_size": n, "x": x, "u_norm": f[0:3].T.copy(),
+            "u_sup": f[3:6].T.copy()}
+
+
+def write_mesh_dat(path, t: float, mesh_size: int, box_size: float, u_sup, u_norm=None) -> None:
+    """print_mesh's file for the grid of mesh_points(mesh_size, box_size)."""
+    n = int(mesh_size)
+    pts = mesh_points(n, box_size)
+    u_sup = np.asarray(u_sup, dtype="<f8").reshape(n ** 3, 3)
+    u_norm = np.zeros_like(u_sup) if u_norm is None else np.asarray(u_norm, "<f8").reshape(-1, 3)
+    # mesh(n/2, n/2, 1:n)%x(1): x_i over i
+    xs = pts[(n // 2 - 1) + n * (n // 2 - 1) + n * n * np.arange(n), 0]
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<d", t))
+        np.ascontiguousarray(xs, dtype="<f8").tofile(fh)
+        for a in (u_norm, u_sup):
+            for c in range(3):
+                np.ascontiguousarray(a[:, c], dtype="<f8").tofile(fh)

@@ -18,11 +18,22 @@
 !> recon_shots divides itime the neighbour search on the device tree,
 !> precon (recon_type 'original') and premove; they run on the host through
 !> the library (qv_line_*), the state going down and back up around them.
+!> With mesh_size > 0 the velocity on setup_mesh's grid (initial.f90:541-574)
+!> is summed after the motion when mesh_shots divides itime and itime >
+!> mesh_print_delay (run.f90:56-58, mesh_velocity timestep.f90:374-452) by
+!> qv_probe_velocity, and data/mesh%03d.dat is written as print_mesh does
+!> (output.f90:305-321) when shots and mesh_shots divide itime, numbered
+!> (itime - mesh_print_delay)/mesh_shots (output.f90:32-41).  u_norm is 0:
+!> the driver has no normal fluid (normal_velocity zero).  In Tree mode the
+!> mesh sums carry Gamma/(4 pi); mesh_reference_scale T (this driver's own
+!> label) keeps the factors of tree_walk_general (tree.f90:460,479) so that
+!> the files match the reference's.
 !>
 !> run.in labels used: nsteps shots dt delta box_size velocity boundary
 !> quant_circ corea tree_theta tree_extra_correction deriv_order runge_kutta initf pcount
-!> tree_print restart_rewind recon_shots, and gpu_precision (fp64 | fp32c) and
-!> line_ops (T | F), this driver's own.
+!> tree_print restart_rewind recon_shots mesh_size mesh_shots mesh_print_delay
+!> normal_velocity, and gpu_precision (fp64 | fp32c), line_ops (T | F) and
+!> mesh_reference_scale (T | F), this driver's own.
 !>
 !> var.dat holds f(:) as the reference's type qvort (cdata.f90:20-38); the
 !> components this driver does not carry (x_interp, acc, u_mf, u_s_LI,
@@ -57,7 +68,13 @@ program qv_run
   character(len=30) :: initf = 'single_loop', gpu_precision = 'fp64'
   logical :: tree_print = .false., restart_rewind = .false., line_ops = .false.
   integer :: recon_shots = 1
+  integer :: mesh_size = 0, mesh_shots = 100, mesh_print_delay = 0
+  logical :: mesh_reference_scale = .false.
+  character(len=30) :: normal_velocity = 'zero'
   type(qv_line_counts) :: counts
+  ! the mesh points (3, mesh_size**3) in the order of mesh(k,j,i) and their
+  ! superfluid velocity
+  real(c_double), allocatable, target :: mesh_x(:,:), mesh_u(:,:)
   ! type qvort (cdata.f90:20-38), component for component: the record layout
   ! of f(:) in data/var.dat
   type :: qvort_record
@@ -90,6 +107,7 @@ program qv_run
 
   call read_run_in()
   call init_filament()
+  call setup_mesh()
 
   p%quant_circ = quant_circ
   p%corea = corea
@@ -146,6 +164,9 @@ program qv_run
         call print_tree_mesh(itime / shots)
       call qv_check(qv_ab_step(ctx), 'qv_ab_step')
     end if
+    if (mesh_size > 0) then
+      if (mod(itime, mesh_shots) == 0 .and. itime > mesh_print_delay) call mesh_velocity()
+    end if
     if (line_ops) call line_step()
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
     t = t + dt
@@ -155,6 +176,9 @@ program qv_run
       call data_dump()
       call print_info(itime / shots)
       call printf(itime / shots)
+      if (mesh_size > 0 .and. mod(itime, mesh_shots) == 0) then
+        if (itime > mesh_print_delay) call print_mesh((itime - mesh_print_delay) / mesh_shots)
+      end if
     end if
   end do
   call qv_check(qv_get_timings(ctx, tm), 'qv_get_timings')
@@ -225,6 +249,16 @@ contains
           read(buffer, *, iostat=ios) recon_shots
         case ('line_ops')
           read(buffer, *, iostat=ios) line_ops
+        case ('mesh_size')
+          read(buffer, *, iostat=ios) mesh_size
+        case ('mesh_shots')
+          read(buffer, *, iostat=ios) mesh_shots
+        case ('mesh_print_delay')
+          read(buffer, *, iostat=ios) mesh_print_delay
+        case ('mesh_reference_scale')
+          read(buffer, *, iostat=ios) mesh_reference_scale
+        case ('normal_velocity')
+          read(buffer, *, iostat=ios) normal_velocity
       end select
       ios = 0
     end do
@@ -234,11 +268,8 @@ contains
     if (velocity == 'Tree') then
       if (tree_theta < epsilon(0._c_double)) call stop_with('init.mod:init_setup', &
         'runnning with tree velocity but tree_theta is zero')
-      if (tree_theta > 0.6_c_double) then
-        write (*,*) '-------------------------WARNING----------------------------'
-        write (*,*) 'init_setup', ": ", 'tree_theta>0.6, the vortices may be unstable'
-        write (*,*) '------------------------------------------------------------'
-      end if
+      if (tree_theta > 0.6_c_double) call warning('init_setup', &
+        'tree_theta>0.6, the vortices may be unstable')
     end if
   end subroutine
 
@@ -462,6 +493,84 @@ contains
     end if
     call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
     call qv_check(qv_upload_history(ctx, c_loc(u1), c_loc(u2), c_loc(u3)), 'qv_upload_history')
+  end subroutine
+
+  !> setup_mesh (initial.f90:541-574) with its checks and messages, and the
+  !> warning of initial.f90:285-286
+  subroutine setup_mesh()
+    integer :: i, j, k, q
+    real(c_double) :: mesh_delta
+    if (mesh_size <= 0) then
+      write(*,*) 'velocity fields not being stored on a mesh - (no spectra etc.)'
+      return
+    end if
+    if (mod(mesh_size, 2) /= 0) call stop_with('init.mod:setup_mesh', &
+      'mesh size must be a multiple of 2')
+    if (mesh_size < 16) call warning('setup_mesh', 'warning mesh size is small')
+    if (mesh_size > 1290) call stop_with('setup_mesh', &
+      'mesh_size**3 points must fit a 32-bit count')
+    if (normal_velocity /= 'zero') call stop_with('setup_mesh', &
+      'the driver has no normal fluid: normal_velocity must be zero')
+    mesh_delta = box_size / mesh_size
+    write(*,'(a,i3.2,a,f7.6)') ' creating an n^3 mesh, n=', mesh_size, ' resolution=', mesh_delta
+    write(*,'(a,i5.2,a)') ' mesh information will be printed every ', mesh_shots, ' time-steps'
+    if (mod(mesh_print_delay, mesh_shots) == 0) then
+      if (mesh_print_delay > 0) write(*,'(a,i5.2,a)') ' will not start printing until itime= ', &
+        mesh_print_delay
+    else
+      call stop_with('init.mod:setup_mesh', 'mesh_print_delay must be a multiply of mesh_shots')
+    end if
+    if (mesh_shots < shots) call warning('init.mod', &
+      'mesh shots < shots which will create output anomolies')
+    allocate(mesh_x(3, mesh_size**3), mesh_u(3, mesh_size**3))
+    ! mesh(k,j,i) at (x_i, y_j, z_k), k fastest as print_mesh writes the arrays
+    q = 0
+    do i = 1, mesh_size
+      do j = 1, mesh_size
+        do k = 1, mesh_size
+          q = q + 1
+          mesh_x(1, q) = mesh_delta * real(2*i-1, c_double) / 2._c_double - (box_size / 2._c_double)
+          mesh_x(2, q) = mesh_delta * real(2*j-1, c_double) / 2._c_double - (box_size / 2._c_double)
+          mesh_x(3, q) = mesh_delta * real(2*k-1, c_double) / 2._c_double - (box_size / 2._c_double)
+        end do
+      end do
+    end do
+    mesh_u = 0._c_double
+  end subroutine
+
+  subroutine warning(location, message)
+    character(len=*), intent(in) :: location, message
+    write (*,*) '-------------------------WARNING----------------------------'
+    write (*,*) trim(location), ": ", trim(message)
+    write (*,*) '------------------------------------------------------------'
+  end subroutine
+
+  !> mesh_velocity (timestep.f90:374-452): the superfluid velocity at the
+  !> mesh points from the step's positions, ghosts and tree
+  subroutine mesh_velocity()
+    integer(c_int32_t) :: flags
+    flags = 0
+    if (velocity == 'Tree' .and. mesh_reference_scale) flags = QV_PROBE_REFERENCE_SCALE
+    call qv_check(qv_probe_velocity(ctx, int(mesh_size**3, c_int32_t), mesh_x, mesh_u, &
+                                    c_null_ptr, flags), 'qv_probe_velocity')
+  end subroutine
+
+  !> data/mesh%03d.dat as print_mesh writes it (output.f90:311-321); u_norm = 0
+  subroutine print_mesh(filenumber)
+    integer, intent(in) :: filenumber
+    character(len=40) :: print_file
+    integer :: fh, i, m
+    m = mesh_size
+    write(print_file, '(a,i3.3,a)') './data/mesh', filenumber, '.dat'
+    open(newunit=fh, file=print_file, form='unformatted', status='replace', access='stream')
+    write(fh) t
+    ! mesh(m/2, m/2, 1:m)%x(1)
+    write(fh) (mesh_x(1, m/2 + m*(m/2 - 1) + m*m*(i - 1)), i = 1, m)
+    write(fh) (0._c_double, i = 1, 3 * m**3)
+    write(fh) mesh_u(1, :)
+    write(fh) mesh_u(2, :)
+    write(fh) mesh_u(3, :)
+    close(fh)
   end subroutine
 
   !> the ts.log line of print_info (output.f90:55-77): recon and removed are

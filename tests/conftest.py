@@ -36,6 +36,11 @@ def fmt_names():
         return sorted(json.load(fh))
 
 
+def mesh_names():
+    with open(os.path.join(GOLDEN, "mesh_index.json")) as fh:
+        return sorted(json.load(fh))
+
+
 def probe_names():
     with open(os.path.join(GOLDEN, "probe_index.json")) as fh:
         return sorted(json.load(fh))
@@ -66,18 +71,20 @@ class Fil:
         return self.x.shape[0]
 
 
+# run.in labels of the time loop and its output, not of the velocity
+_LOOP_KEYS = ("runge_kutta", "shots", "mesh_size", "mesh_shots", "mesh_print_delay")
+
+
 def oracle_params(p):
     from oracle import port
-    kw = dict(p)
-    kw.pop("runge_kutta", None)
+    kw = {k: v for k, v in p.items() if k not in _LOOP_KEYS}
     kw["deriv_order"] = 4 if kw.get("deriv_order") == "fourth" else 2
     return port.make_params(**kw)
 
 
 def gpu_params(p, precision="fp64"):
     from qvort_amd import gpu
-    kw = dict(p)
-    kw.pop("runge_kutta", None)
+    kw = {k: v for k, v in p.items() if k not in _LOOP_KEYS}
     kw["deriv_order"] = 4 if kw.get("deriv_order") == "fourth" else 2
     return gpu.params(precision=precision, **kw)
 

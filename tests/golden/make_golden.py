@@ -11,6 +11,7 @@ case records the run.in parameters it used.  Run from the repository root:
     python tests/golden/make_golden.py --rk3        # pmotion_RK3 trajectories
     python tests/golden/make_golden.py --formats    # var.dat and tree_mesh files
     python tests/golden/make_golden.py --line       # pinsert / precon / premove
+    python tests/golden/make_golden.py --mesh       # mesh velocity files
 
 The neighbour-search cases (closest_*.npz, index in closest_index.json) run
 one step of ghostp, construct_tree, pmotion, optionally pinsert, then
@@ -35,6 +36,11 @@ statements).  They store the input state, the var.dat bytes after nsteps,
 the positions and AB history after nsteps, the text of every
 data/tree_mesh%03d.log (mesh_<n>), and from the restarted run the positions
 after the extra step and its var.dat (var_dat_restart).
+
+The mesh cases (mesh_*.npz, index in mesh_index.json) run nsteps of the main
+loop with mesh_size > 0: the reference's mesh_velocity after the motion and
+data/mesh%03d.dat as print_mesh writes it.  They store the input state and
+the bytes of every mesh file (mesh_<n>).
 """
 from __future__ import annotations
 
@@ -272,7 +278,48 @@ def main_formats():
         json.dump(index, fh, indent=1, sort_keys=True)
 
 
+def mesh_cases():
+    holes = G.with_holes(G.random_loops(0.1, 5e-4, n_loops=3, seed=91), 20, seed=9)
+    loops = G.random_loops(0.1, 5e-4, n_loops=3, seed=92)
+    base = dict(delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4, box_size=0.1, shots=1)
+    return [
+        ("mesh_holes_bs_periodic", holes, dict(base, velocity="BS", tree_theta=0.0,
+                                               boundary="periodic", mesh_size=6,
+                                               mesh_shots=1), 2),
+        # the first print waits for itime > 2; files are numbered from there
+        ("mesh_loops_tree_openx", loops, dict(base, velocity="Tree", tree_theta=0.4,
+                                              boundary="openx", mesh_size=8, mesh_shots=2,
+                                              mesh_print_delay=2), 4),
+        ("mesh_loops_rk3_tree_periodic", loops, dict(base, velocity="Tree", tree_theta=0.5,
+                                                     boundary="periodic", mesh_size=4,
+                                                     mesh_shots=1, runge_kutta=True), 2),
+        ("mesh_holes_bs_openxy", holes, dict(base, velocity="BS", tree_theta=0.0,
+                                             boundary="openxy", mesh_size=4, mesh_shots=1,
+                                             mesh_print_delay=1), 2),
+    ]
+
+
+def main_mesh():
+    index = {}
+    for name, fil, params, nsteps in mesh_cases():
+        r = refrun.run(fil, nsteps=nsteps, threads=1, **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind, x_final=r["x"])
+        grids = r.get("mesh", {})
+        for k, raw in grids.items():
+            arrays[f"mesh_{k}"] = np.frombuffer(raw, dtype=np.uint8)
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount, meshes=sorted(grids))
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "meshes", sorted(grids), flush=True)
+    with open(os.path.join(OUT, "mesh_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def main():
+    if "--mesh" in sys.argv[1:]:
+        main_mesh()
+        return
     if "--line" in sys.argv[1:]:
         main_line()
         return

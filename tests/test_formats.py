@@ -8,13 +8,22 @@ element), and every data/tree_mesh%03d.log construct_tree printed.  The
 layout is pinned by finding those independent arrays at their offsets, the
 writer by byte equality and by a restart of the reference build from a
 file this module wrote, the leaf print by the oracle's own tree.
+
+The mesh fixtures (tests/golden/mesh_*.npz) hold every data/mesh%03d.dat the
+reference build wrote with mesh_size > 0: the reader finds the grid planes
+of setup_mesh and zero normal velocity, the writer reproduces the bytes, and
+for the Adams-Bashforth runs the oracle's probe sums at mesh_points after
+replaying the steps reproduce u_sup bit for bit, which pins the point order
+and the state the sums see (positions after the motion, before the wrap;
+ghosts and tree of the step's start).
 """
 import numpy as np
 import pytest
 
-from conftest import Fil, fmt_names, load_golden, oracle_params
+from conftest import Fil, fmt_names, load_golden, mesh_names, oracle_params
 
 NAMES = fmt_names()
+MESH_NAMES = mesh_names()
 
 
 def _t_after(nsteps, dt):
@@ -127,3 +136,104 @@ def test_var_log_round_trip(tmp_path):
     assert np.array_equal(d["x"], x) and np.array_equal(d["u"], u)
     assert np.array_equal(d["infront"], inf)
     assert path.stat().st_size == 12 + n * (14 * 8 + 4)
+
+
+def _mesh_files(g, tmp_path):
+    meta = g["meta"]
+    out = {}
+    for k in meta["meshes"]:
+        path = tmp_path / f"mesh{k:03d}.dat"
+        path.write_bytes(g[f"mesh_{k}"].tobytes())
+        out[k] = path
+    return out
+
+
+def _mesh_itime(meta, k):
+    p = meta["params"]
+    return p.get("mesh_print_delay", 0) + k * p["mesh_shots"]
+
+
+@pytest.mark.parametrize("name", MESH_NAMES)
+def test_mesh_dat_layout_and_writer(name, tmp_path):
+    from qvort_amd import formats
+    g = load_golden(name)
+    meta = g["meta"]
+    p = meta["params"]
+    n = p["mesh_size"]
+    # print_mesh writes when shots and mesh_shots divide itime and itime >
+    # mesh_print_delay, numbered (itime - delay) / mesh_shots
+    expect = [(it - p.get("mesh_print_delay", 0)) // p["mesh_shots"]
+              for it in range(1, meta["nsteps"] + 1)
+              if it % p["mesh_shots"] == 0 and it % p["shots"] == 0
+              and it > p.get("mesh_print_delay", 0)]
+    assert meta["meshes"] == expect
+    pts = formats.mesh_points(n, p["box_size"])
+    for k, path in _mesh_files(g, tmp_path).items():
+        d = formats.read_mesh_dat(path)
+        assert d["mesh_size"] == n
+        assert d["t"] == _t_after(_mesh_itime(meta, k), p["dt"])
+        assert np.array_equal(d["x"], np.unique(pts[:, 0]))
+        assert not np.any(d["u_norm"])
+        out = tmp_path / "w.dat"
+        formats.write_mesh_dat(out, d["t"], n, p["box_size"], d["u_sup"], d["u_norm"])
+        assert out.read_bytes() == path.read_bytes()
+
+
+@pytest.mark.parametrize("name", [n for n in MESH_NAMES if "rk3" not in n])
+def test_mesh_velocity_matches_oracle_probe_sums(name, oracle_lib, tmp_path):
+    from oracle import port
+    from qvort_amd import formats
+    g = load_golden(name)
+    meta = g["meta"]
+    p = oracle_params(meta["params"])
+    n = meta["params"]["mesh_size"]
+    pts = formats.mesh_points(n, meta["params"]["box_size"])
+    for k, path in _mesh_files(g, tmp_path).items():
+        it = _mesh_itime(meta, k)
+        fil = Fil(g["x0"], g["infront"], g["behind"])
+        x, u1, u2, u3, _ = port.run(p, fil, it - 1)
+        start = Fil(x, g["infront"], g["behind"])
+        u = port.velocity(p, start)[0]
+        gi = port.ghostp(p, start)[0]
+        moved = port.ab_update(p, x, g["infront"], u, u1, u2, u3)[0]
+        up = port.probe_velocity(p, start, Fil(moved, g["infront"], g["behind"]), gi, pts,
+                                 ref_scaling=True)
+        assert np.array_equal(up, formats.read_mesh_dat(path)["u_sup"])
+
+
+def test_mesh_points_rejects_odd_size():
+    from qvort_amd import formats
+    with pytest.raises(ValueError):
+        formats.mesh_points(5, 0.1)
+    with pytest.raises(formats.FormatError):
+        formats._mesh_size_from_bytes(8 + 8 * 4 + 48 * 64 + 8)
+
+
+@pytest.mark.parametrize("labels, message", [
+    (["mesh_size 7"], "mesh size must be a multiple of 2"),
+    (["mesh_size 8", "mesh_shots 3", "mesh_print_delay 2"],
+     "mesh_print_delay must be a multiply of mesh_shots"),
+    (["mesh_size 8", "normal_velocity xflow"], "normal_velocity must be zero"),
+])
+def test_driver_mesh_setup_errors(labels, message, tmp_path):
+    """setup_mesh's fatal errors stop the Fortran driver before any device
+    call (so this runs without a GPU)."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    driver = os.path.join(ROOT, "qvort_amd", "lib", "qv_run")
+    if not os.path.exists(driver):
+        pytest.skip("qv_run not built")
+    g = load_golden(MESH_NAMES[0])
+    n = g["x0"].shape[0]
+    with open(tmp_path / "state.bin", "wb") as fh:
+        np.array([1, n, 1, 0, 0], dtype="<i4").tofile(fh)
+        g["x0"].astype("<f8").tofile(fh)
+        g["infront"].astype("<i4").tofile(fh)
+        g["behind"].astype("<i4").tofile(fh)
+        np.zeros((3, n, 3)).tofile(fh)
+    (tmp_path / "run.in").write_text(
+        "\n".join(["nsteps 1", "box_size 0.1", "velocity BS", "initf state_file"] + labels)
+        + "\n")
+    r = subprocess.run([driver], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and message in r.stdout
