@@ -53,6 +53,9 @@ def lib():
         L.qvo_velocity_targets.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, i32, C.c_int32,
                                            d, i64, C.c_int32, d, d]
         L.qvo_velocity_targets.restype = C.c_int
+        L.qvo_leaf_sum_targets.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, i32, C.c_int32,
+                                           d, i64, C.c_int32]
+        L.qvo_leaf_sum_targets.restype = C.c_int
         L.qvo_tree_dump.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, C.c_int64]
         L.qvo_tree_dump.restype = C.c_int64
         L.qvo_ghostp.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, d]
@@ -120,6 +123,21 @@ def velocity_targets(params: QvoParams, fil, targets, nthreads: int = 0):
     if rc != 0:
         raise RuntimeError(f"qvo_velocity_targets failed ({rc})")
     return u, int(ev.value), bs.value, ws.value
+
+
+def leaf_sum_targets(params: QvoParams, fil, targets, nthreads: int = 0):
+    """(u, evals): the theta -> 0+ limit of the Tree velocity (every leaf
+    with tree_walk's exclusions, summed in slot order) for the listed
+    targets -- the reference of the multipole truncation error."""
+    x, inf, beh = _prep(fil)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    u = np.zeros((t.shape[0], 3))
+    ev = C.c_int64(0)
+    rc = lib().qvo_leaf_sum_targets(C.byref(params), x.shape[0], _d(x), _i(inf), _i(beh), _i(t),
+                                    t.shape[0], _d(u), C.byref(ev), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"qvo_leaf_sum_targets failed ({rc})")
+    return u, int(ev.value)
 
 
 def tree_dump(params: QvoParams, fil) -> np.ndarray:

@@ -363,6 +363,28 @@ static int construct_tree(const qvo_state *s, qvo_tree *t) {
 }
 
 /* tree_walk, tree.f90:336-421.  Returns nonzero on the singularity error. */
+/* the leaf term of tree_walk (tree.f90:377-391): segment x_p -> ghosti_p,
+ * shifted, seen from x_i */
+static void leaf_term(const qvo_state *s, int p, int i, const double *shift, double *u) {
+  const double *xi = s->x + 3 * i;
+  const double *xp = s->x + 3 * p, *gp = s->gi + 3 * p;
+  double R1[3], R2[3], ub[3];
+  for (int k = 0; k < 3; ++k) {
+    R1[k] = (xp[k] + shift[k]) - xi[k];
+    R2[k] = (gp[k] + shift[k]) - xi[k];
+  }
+  const double a = vnorm(R1), b = vnorm(R2);
+  const double c = (a * b) * ((a * b) + dot3(R1, R2));
+  cross(R1, R2, ub);
+  const double f = (a + b) / c;
+  const double q = s->p->quant_circ;
+  for (int k = 0; k < 3; ++k) {
+    ub[k] = f * ub[k];
+    ub[k] = ((ub[k] * q) / 4.) / QVO_PI;
+    u[k] = u[k] + ub[k];
+  }
+}
+
 static int tree_walk(const qvo_state *s, const qvo_tree *t, int64_t ni, int i,
                      const double *shift, double *u, int64_t *evals) {
   const qvo_node *nd = &t->n[ni];
@@ -389,22 +411,7 @@ static int tree_walk(const qvo_state *s, const qvo_tree *t, int64_t ni, int i,
     if (j == s->behind[i] - 1) return 0;
     if (j == s->infront[i] - 1) return 0;
     if (dist < 2.220446049250313e-16) return 1;       /* :372-376 */
-    const double *xp = s->x + 3 * p, *gp = s->gi + 3 * p;
-    double R1[3], R2[3], ub[3];
-    for (int k = 0; k < 3; ++k) {
-      R1[k] = (xp[k] + shift[k]) - xi[k];
-      R2[k] = (gp[k] + shift[k]) - xi[k];
-    }
-    const double a = vnorm(R1), b = vnorm(R2);
-    const double c = (a * b) * ((a * b) + dot3(R1, R2));
-    cross(R1, R2, ub);
-    const double f = (a + b) / c;
-    const double q = s->p->quant_circ;
-    for (int k = 0; k < 3; ++k) {
-      ub[k] = f * ub[k];
-      ub[k] = ((ub[k] * q) / 4.) / QVO_PI;
-      u[k] = u[k] + ub[k];
-    }
+    leaf_term(s, p, i, shift, u);
     *evals += 1;
   } else if (theta < s->p->tree_theta) {
     double v[3], ub[3];
@@ -605,6 +612,64 @@ int qvo_velocity_targets(const qvo_params *p, int32_t pcount, const double *x,
   free(t.n);
   state_free(&s);
   return err ? -2 : 0;
+}
+
+/* The theta -> 0+ limit of the Tree velocity for the listed targets: LIA +
+ * the leaf term of every active slot inside the root cell (the tree's
+ * membership test, tree.f90:146-150, on the root bounds) with tree_walk's
+ * exclusions (j = i, behind(i), infront(i)), central shell then the image
+ * shells composed as calc_velocity does (timestep.f90:268-288).  Sources
+ * are summed in slot order, not in the tree's DFS order, so the result
+ * equals a theta -> 0+ walk up to round-off: it is the reference for the
+ * truncation error of the multipole acceptance, not a bit-exact restatement.
+ * u[3*k] for target k; evals counts leaf terms. */
+int qvo_leaf_sum_targets(const qvo_params *p, int32_t pcount, const double *x,
+                         const int32_t *infront, const int32_t *behind,
+                         const int32_t *targets, int32_t ntargets, double *u,
+                         int64_t *evals, int32_t nthreads) {
+  if (check_params(p, pcount)) return -1;
+  qvo_state s;
+  if (state_init(&s, p, pcount, x, infront, behind)) { state_free(&s); return -1; }
+  const double lo = -p->box_size / 2.;
+  char *inbox = (char *)malloc((size_t)pcount);
+  if (!inbox) { state_free(&s); return -1; }
+  for (int j = 0; j < pcount; ++j) {
+    const double *xj = x + 3 * j;
+    int in = infront[j] != 0;
+    for (int k = 0; k < 3; ++k) in = in && xj[k] >= lo && xj[k] < lo + p->box_size;
+    inbox[j] = (char)in;
+  }
+  double sh[27][3];
+  const int ns = shell_shifts(p, sh);
+  int64_t ev = 0;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : ev)
+  for (int k = 0; k < ntargets; ++k) {
+    double ui[3] = {0., 0., 0.};
+    const int i = targets[k];
+    if (i >= 0 && i < pcount && infront[i] != 0) {
+      const int bi = behind[i] - 1, fi = infront[i] - 1;
+      lia(p, s.x + 3 * i, s.gi + 3 * i, s.gb + 3 * i, s.gii + 3 * i, s.gbb + 3 * i, ui);
+      double ub[3] = {0., 0., 0.}, ui2[3] = {0., 0., 0.};
+      for (int q = 0; q < ns; ++q) {
+        double *acc = q == 0 ? ub : ui2;
+        for (int j = 0; j < pcount; ++j) {
+          if (!inbox[j]) continue;
+          const int jj = behind[infront[j] - 1] - 1;
+          if (jj == i || jj == bi || jj == fi) continue;
+          leaf_term(&s, j, i, sh[q], acc);
+          ++ev;
+        }
+      }
+      for (int c = 0; c < 3; ++c) ui[c] = ui[c] + ub[c];
+      if (ns > 1) for (int c = 0; c < 3; ++c) ui[c] = ui[c] + ui2[c];
+    }
+    for (int c = 0; c < 3; ++c) u[3 * k + c] = ui[c];
+  }
+  if (evals) *evals = ev;
+  free(inbox);
+  state_free(&s);
+  return 0;
 }
 
 static void dump_rec(const qvo_state *s, const qvo_tree *t, int64_t ni,

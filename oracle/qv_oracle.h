@@ -97,6 +97,14 @@ int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
                 const int32_t *infront, const int32_t *behind, double *u1,
                 int32_t nsteps, double *u_last, int32_t nthreads);
 
+/* theta -> 0+ limit of the Tree velocity (every leaf with tree_walk's
+ * exclusions, slot order) for the listed targets: the reference for the
+ * multipole truncation error.  u[3*k] for target k. */
+int qvo_leaf_sum_targets(const qvo_params *p, int32_t pcount, const double *x,
+                         const int32_t *infront, const int32_t *behind,
+                         const int32_t *targets, int32_t ntargets, double *u,
+                         int64_t *evals, int32_t nthreads);
+
 /* neighbour search of the reconnection step.
  * qvo_closest_tree: pclose_tree / closest_tree (tree.f90:219-278).  The tree
  * is built from the build state (x_b, infront_b, behind_b: the positions and

@@ -141,6 +141,24 @@ def test_tree_small_theta_is_direct_sum(oracle_lib):
     assert ev == 2 * n + 27 * n * (n - 3) + 26 * 3 * n - 26 * 3 * n  # every leaf, 3 excluded
 
 
+def test_leaf_sum_is_small_theta_tree(oracle_lib):
+    """The leaf sum (the truncation-error reference) equals the theta -> 0+
+    walk up to round-off, with the same number of leaf terms, and the
+    numpy brute force of the tree's exclusion rule."""
+    from oracle import port
+    fil = G.random_loops(0.1, 5e-4, n_loops=2, seed=22)
+    L = 0.1
+    p = _params(velocity="Tree", tree_theta=1e-9, boundary="periodic", box_size=L)
+    u, uli, ubs, ev = port.velocity(p, fil)
+    act = np.flatnonzero(fil.infront).astype(np.int32)
+    ul, evl = port.leaf_sum_targets(p, fil, act)
+    assert np.max(rel_err(ul, u[act])) < 1e-11
+    assert evl == ev - 2 * act.size
+    shells = [np.array([a, b, c]) * L for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+    ref = _direct_tree_rule(fil, 9.97e-4, L, shells)
+    assert np.max(rel_err(ul - uli[act], ref[act])) < 1e-11
+
+
 def test_bs_brute_force_tiny(oracle_lib):
     """Direct sum (biot_savart + 26 shifted copies incl. the self image,
     timestep.f90:462-507) against an independent numpy evaluation."""
