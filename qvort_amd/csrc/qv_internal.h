@@ -100,6 +100,9 @@ struct Ctx {
   // tree build
   uint64_t *key0 = nullptr, *key1 = nullptr, *key0_s = nullptr;
   int32_t *slot = nullptr, *slot_s = nullptr;  // sorted active slots
+  // the walk's target order: sorted positions in Hilbert-curve order, so
+  // that the 64 targets of a wavefront are spatially compact
+  int32_t *walk_order = nullptr;
   int32_t *nnode_of = nullptr, *node_off = nullptr;  // per sorted position
   Vec3 xs, gs;  // positions and ghosts in sorted order
   void *sort_tmp = nullptr;

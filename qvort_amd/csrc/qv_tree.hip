@@ -498,6 +498,54 @@ __global__ void __launch_bounds__(kBlock) k_moments_big(MomArgs A) {
   }
 }
 
+// 30-bit Hilbert-curve index of each sorted position on a 1024^3 grid of the
+// box (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 381 (2004): axes to transposed index, then bit interleave); the index
+// itself is the sort's value.  Only the walk's grouping of targets into
+// wavefronts uses this order: each target's own walk and sums do not
+// depend on it.
+__global__ void __launch_bounds__(kBlock) k_hilbert_keys(int nact, double L,
+                                                         const double *__restrict__ xs,
+                                                         const double *__restrict__ ys,
+                                                         const double *__restrict__ zs,
+                                                         uint32_t *__restrict__ key,
+                                                         int32_t *__restrict__ idx) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nact) return;
+  constexpr int kBits = 10;
+  const double scale = (double)(1 << kBits) / L;
+  const double p[3] = {xs[s], ys[s], zs[s]};
+  uint32_t X[3];
+  for (int k = 0; k < 3; ++k) {
+    const double q = (p[k] + L / 2.) * scale;
+    X[k] = q <= 0. ? 0u : (q >= (double)((1 << kBits) - 1) ? (uint32_t)((1 << kBits) - 1)
+                                                              : (uint32_t)q);
+  }
+  for (uint32_t Q = 1u << (kBits - 1); Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t Q = 1u << (kBits - 1); Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  X[0] ^= t; X[1] ^= t; X[2] ^= t;
+  uint32_t h = 0;
+  for (int b = kBits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+  key[s] = h;
+  idx[s] = s;
+}
+
 hipError_t grow_tree(Tree &t, int64_t need) {
   if (need <= t.cap) return hipSuccess;
   int64_t cap = need + need / 4 + 1024;
@@ -561,6 +609,28 @@ int tree_build(Ctx &c) {
                      c.key1, key1_s, c.x.x, c.x.y, c.x.z, c.gi.x, c.gi.y, c.gi.z, c.xs.x, c.xs.y,
                      c.xs.z, c.gs.x, c.gs.y, c.gs.z);
   QV_TRY(hipGetLastError());
+  // 4b. the walk's order of the sorted targets (Hilbert curve); the
+  // unsorted key and slot buffers are free from here on
+  if (nact > 0) {
+    uint32_t *hk = reinterpret_cast<uint32_t *>(c.key0);
+    uint32_t *hk_s = hk + c.cap;  // key0 holds 2 cap 32-bit words
+    hipLaunchKernelGGL(k_hilbert_keys, dim3(grid_of(nact)), dim3(kBlock), 0, st, nact,
+                       c.p.box_size, c.xs.x, c.xs.y, c.xs.z, hk, c.slot);
+    QV_TRY(hipGetLastError());
+    size_t hneed = 0;
+    QV_TRY(rocprim::radix_sort_pairs(nullptr, hneed, hk, hk_s, c.slot, c.walk_order, nact, 0, 30,
+                                     st));
+    if (hneed > c.sort_tmp_bytes) {
+      (void)hipFree(c.sort_tmp);
+      c.sort_tmp = nullptr;
+      c.sort_tmp_bytes = 0;
+      QV_TRY(hipMalloc(&c.sort_tmp, hneed));
+      c.sort_tmp_bytes = hneed;
+    }
+    size_t hhave = c.sort_tmp_bytes;
+    QV_TRY(rocprim::radix_sort_pairs(c.sort_tmp, hhave, hk, hk_s, c.slot, c.walk_order, nact, 0,
+                                     30, st));
+  }
   // 5. nodes per sorted position, prefix sum -> DFS preorder offsets
   QV_TRY(hipMemsetAsync(c.nnode_of + nact, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_count_nodes, dim3(grid_of(nact)), dim3(kBlock), 0, st, nact, n, c.key0_s,
