@@ -29,6 +29,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include "qv_sort.h"
+
 #include "qv_internal.h"
 
 #pragma clang fp contract(off)
@@ -436,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
     return;
   }
   for (int k = 0; k < 15; ++k) s[k] = 0.;
+#pragma unroll 4
   for (int t = f; t < f + n; ++t) add_point(A, t, s);
   if (n == 0) {
     NodeRec r;
@@ -498,8 +501,9 @@ __global__ void __launch_bounds__(kBlock) k_moments_big(MomArgs A) {
   }
 }
 
-// 30-bit Hilbert-curve index of each sorted position on a 1024^3 grid of the
-// box (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 21-bit Hilbert-curve index of each sorted position on a 128^3 grid of the
+// box (finer grids group the 1M-point workload no better; ties keep the
+// Morton order of the sorted positions) (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
 // 381 (2004): axes to transposed index, then bit interleave); the index
 // itself is the sort's value.  Only the walk's grouping of targets into
 // wavefronts uses this order: each target's own walk and sums do not
@@ -512,7 +516,7 @@ __global__ void __launch_bounds__(kBlock) k_hilbert_keys(int nact, double L,
                                                          int32_t *__restrict__ idx) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nact) return;
-  constexpr int kBits = 10;
+  constexpr int kBits = 7;
   const double scale = (double)(1 << kBits) / L;
   const double p[3] = {xs[s], ys[s], zs[s]};
   uint32_t X[3];
@@ -618,8 +622,8 @@ int tree_build(Ctx &c) {
                        c.p.box_size, c.xs.x, c.xs.y, c.xs.z, hk, c.slot);
     QV_TRY(hipGetLastError());
     size_t hneed = 0;
-    QV_TRY(rocprim::radix_sort_pairs(nullptr, hneed, hk, hk_s, c.slot, c.walk_order, nact, 0, 30,
-                                     st));
+    QV_TRY(rocprim::radix_sort_pairs<SortConfig>(nullptr, hneed, hk, hk_s, c.slot, c.walk_order,
+                                                 nact, 0, 21, st));
     if (hneed > c.sort_tmp_bytes) {
       (void)hipFree(c.sort_tmp);
       c.sort_tmp = nullptr;
@@ -628,8 +632,8 @@ int tree_build(Ctx &c) {
       c.sort_tmp_bytes = hneed;
     }
     size_t hhave = c.sort_tmp_bytes;
-    QV_TRY(rocprim::radix_sort_pairs(c.sort_tmp, hhave, hk, hk_s, c.slot, c.walk_order, nact, 0,
-                                     30, st));
+    QV_TRY(rocprim::radix_sort_pairs<SortConfig>(c.sort_tmp, hhave, hk, hk_s, c.slot, c.walk_order,
+                                                 nact, 0, 21, st));
   }
   // 5. nodes per sorted position, prefix sum -> DFS preorder offsets
   QV_TRY(hipMemsetAsync(c.nnode_of + nact, 0, sizeof(int32_t), st));
