@@ -315,7 +315,7 @@ int qv_destroy(qv_ctx *ctx) {
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
-                c->tree.linf, c->close_i, c->close_d, c->probe_buf};
+                c->tree.linf, c->close_i, c->close_d, c->close_order, c->probe_buf, c->probe_part};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->h_evals) (void)hipHostFree(c->h_evals);
@@ -333,6 +333,7 @@ int qv_set_params(qv_ctx *ctx, const qv_params *params) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   c->p = *params;
   c->have_velocity = false;
+  c->tree.plain_valid = false;  // the probe thresholds depend on tree_theta
   return QV_OK;
 }
 
@@ -493,11 +494,17 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
     const int32_t cap = c->cap;  // the slot buffers' capacity (>= pcount)
     TRY(dalloc(c->close_i, cap));
     TRY(dalloc(c->close_d, cap));
+    TRY(dalloc(c->close_order, cap));
     c->close_cap = cap;
   }
   const size_t n = c->pcount;
   TRY(hipMemcpyAsync(c->close_i, closest, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
   TRY(hipMemcpyAsync(c->close_d, closestd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (tree) {
+    // the search's slot order; the slot buffers hold cap >= pcount entries
+    const int rc = hilbert_order(*c, c->pcount, c->x, c->infront, c->close_order);
+    if (rc) return rc;
+  }
   TRY(launch_closest(*c, tree));
   TRY(hipMemcpyAsync(closest, c->close_i, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   TRY(hipMemcpyAsync(closestd, c->close_d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));

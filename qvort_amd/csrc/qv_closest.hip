@@ -13,8 +13,7 @@
 // empty or not (tree.f90:227).
 //
 // The walk is wavefront-cooperative, as the velocity walk is: the 64 slots
-// of a wavefront (consecutive slots, mostly consecutive points of a loop)
-// share one DFS whose current node is read with scalar loads; each lane
+// of a wavefront (adjacent on a Hilbert curve through the box) share one DFS whose current node is read with scalar loads; each lane
 // takes part only in the nodes of its own walk (its bit of the mask), so it
 // meets them in the same order and updates closest / closestd exactly as
 // its own recursive walk would.
@@ -36,6 +35,7 @@ constexpr int kBlock = 256;
 struct CloseArgs {
   int n;                 // slots of the current state
   int64_t nnodes;
+  const int32_t *order;  // tree mode: the slots in the search's order
   const NodeRec *rec;
   const double2 *geo;    // width
   const int32_t *count;  // points per node (root: all slots)
@@ -60,8 +60,9 @@ __device__ inline uint64_t uni64(uint64_t v) {
 
 __global__ void __launch_bounds__(kBlock) k_closest_tree(CloseArgs A) {
   __shared__ CloseEntry stack[kBlock / kWave][kStackDepth];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < A.n;
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = w < A.n;
+  const int i = valid ? A.order[w] : 0;
   if (valid) A.cd[i] = 100.;
   const bool act = valid && A.inf[i] != 0;
   int32_t fi = 0, bi = 0, best = 0;
@@ -177,6 +178,7 @@ hipError_t launch_closest(Ctx &c, bool tree) {
   CloseArgs A;
   A.n = c.pcount;
   A.nnodes = tree ? c.tree.nnodes : 0;
+  A.order = c.close_order;
   A.rec = c.tree.rec;
   A.geo = c.tree.geo;
   A.count = c.tree.count;

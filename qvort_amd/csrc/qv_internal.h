@@ -59,6 +59,11 @@ struct Tree {
   double *mp = nullptr;     // 12 per node
   float *mp32 = nullptr;    // 16 per node
   double2 *geo = nullptr;   // width, geo_dist
+  // smallest d2 accepted by the probe walk's plain test width/dist < theta
+  // (tree_walk_general), per node; computed on the first probe call after a
+  // build (plain_valid); -1: exact test at every visit
+  double *tplain = nullptr;
+  bool plain_valid = false;
   // build-only arrays (per node)
   int32_t *first = nullptr;  // first sorted position
   int32_t *count = nullptr;  // member count (root: pcount slots)
@@ -85,6 +90,7 @@ struct Ctx {
   bool tree_valid = false;
   int32_t tree_pcount = 0;
   int32_t *close_i = nullptr;   // closest (1-based), per slot
+  int32_t *close_order = nullptr;  // the search's slot order (Hilbert curve)
   double *close_d = nullptr;    // closestd, per slot
   int32_t close_cap = 0;
   // ghosts of the last ghostp (velocity stage or a probe call with
@@ -92,6 +98,8 @@ struct Ctx {
   bool ghosts_valid = false;
   double *probe_buf = nullptr;  // probes (3n), u (3n), gradient (9n), sort keys (3n)
   int32_t probe_cap = 0;
+  double *probe_part = nullptr;  // Tree probes: partial sums per shell chunk
+  size_t probe_part_cap = 0;     // doubles
   // state (SoA), slot order
   Vec3 x, gi, gb, gii, gbb, u, uli, ubs, u1, u2, u3;
   int32_t *infront = nullptr, *behind = nullptr;
@@ -168,6 +176,17 @@ hipError_t launch_ab_update(Ctx &c);
 hipError_t launch_rk3_update(Ctx &c, int stage);
 hipError_t launch_enforce_periodic(Ctx &c);
 hipError_t launch_direct(Ctx &c);
+// 21-bit Hilbert-curve keys of n points (x[i*stride], y[..], z[..]) on a
+// 128^3 grid of the box, idx[i] = i; with inf, points with inf[i] = 0 get
+// the key 2^21
+hipError_t launch_hilbert_keys(hipStream_t st, int n, double L, const double *x, const double *y,
+                               const double *z, int stride, const int32_t *inf, uint32_t *key,
+                               int32_t *idx);
+// the probe walk's per-node thresholds (Tree::tplain) for the current tree
+hipError_t launch_plain_thresholds(Ctx &c);
+// slots 0..n-1 (positions pos) in Hilbert-curve order into order[0..n);
+// with inf, slots with infront = 0 go last.  Uses the build's key buffers.
+int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order);
 int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
 hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);

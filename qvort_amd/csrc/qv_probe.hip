@@ -13,17 +13,26 @@
 //                  differences of velocity_quasip (hamiltonian.f90:64-79).
 //   velocity Tree: tree_walk_general (tree.f90:425-488) over the tree of the
 //                  last velocity stage.  The probes are put in the order of
-//                  their 63-bit Morton keys in the box (a radix sort), and
-//                  the 64 probes of a wavefront share one DFS per shell,
+//                  their 21-bit Hilbert-curve keys in the box (a radix sort),
+//                  and the 64 probes of a wavefront share one DFS per shell,
 //                  node records by scalar loads, each lane taking part only
 //                  in its own walk's nodes, in the same order as its own
-//                  recursive walk.  The reference adds leaves as
+//                  recursive walk.  Fewer than about half a million probes
+//                  cannot fill the GPU that way: their shells are split over
+//                  several wavefronts per group of 64 probes, whose partial
+//                  sums are added in shell order after the walk.  The
+//                  reference adds leaves as
 //                  u_bs/4/pi (no Gamma) and cells without Gamma/(4 pi); by
 //                  default both get Gamma/(4 pi) here, and
 //                  QV_PROBE_REFERENCE_SCALE keeps the reference's factors.
 #include "qv_internal.h"
+#include "qv_terms.h"
+
+#include <algorithm>
 
 #include <rocprim/device/device_radix_sort.hpp>
+
+#include "qv_sort.h"
 
 #pragma clang fp contract(off)
 
@@ -48,7 +57,12 @@ struct ProbeArgs {
   const NodeRec *rec;
   const double2 *geo;
   const int32_t *count;
+  const double *tplain;  // plain-test thresholds on d2 (-1: exact test)
   const double *mp;  // internal: circ(3), C(9) row-major C[i][k]; leaf: ghosti(3)
+  // shells split over nsplit wavefronts per group of 64 probes: partial sums
+  // part[(q * nsplit + c) * 3 + k] of chunk c, added in chunk order after
+  int nsplit;
+  double *part;
 };
 
 // one term of biot_savart_general(_shift) at probe y, source x (+ shift s),
@@ -157,31 +171,6 @@ __global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
     for (int k = 0; k < 9; ++k) A.g[9 * q + k] = G[k];
 }
 
-// 21 bits per axis of the position in the box (clamped), interleaved
-__device__ inline uint64_t spread21(uint64_t v) {
-  v &= 0x1fffff;
-  v = (v | (v << 32)) & 0x1f00000000ffffull;
-  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
-  v = (v | (v << 8)) & 0x100f00f00f00f00full;
-  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
-
-__global__ void __launch_bounds__(kBlock) k_probe_keys(int nprobe, const double *py, double L,
-                                                       uint64_t *key, int32_t *idx) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nprobe) return;
-  uint64_t k = 0;
-  for (int d = 0; d < 3; ++d) {
-    double f = (py[3 * q + d] + L / 2.) / L;
-    f = fmin(fmax(f, 0.), 1. - 1e-12);
-    k |= spread21((uint64_t)(f * 2097152.)) << d;
-  }
-  key[q] = k;
-  idx[q] = q;
-}
-
 struct alignas(16) ProbeEntry {
   int32_t node, end;
   uint64_t mask;
@@ -194,11 +183,14 @@ __device__ inline uint64_t uni64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// lane q of the sorted order works on probe order[q]
+// wavefront gw walks shell chunk gw % nsplit for the 64 probes of group
+// gw / nsplit; lane l of group p works on probe order[64 p + l]
 __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
                                                        const int32_t *__restrict__ order) {
   __shared__ ProbeEntry stack[kBlock / kWave][kStackDepth];
-  const int qs = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gw = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int chunk = gw % A.nsplit;
+  const int qs = (gw / A.nsplit) * kWave + threadIdx.x % kWave;
   const bool act = qs < A.nprobe;
   const int q = act ? order[qs] : 0;
   double y[3] = {0., 0., 0.};
@@ -207,8 +199,11 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
   const uint64_t all = __ballot(act);
   if (all == 0) return;  // the whole wavefront
   ProbeEntry *st = stack[threadIdx.x / kWave];
-  double u[3] = {0., 0., 0.};
-  for (int sh = 0; sh < S.n; ++sh) {
+  // leaf and cell terms in two sums: the factors (Gamma/(4 pi) for both, or
+  // the reference's 1/(4 pi) for leaves and 1 for cells) are applied once
+  double ul[3] = {0., 0., 0.}, uc[3] = {0., 0., 0.};
+  const int sh0 = chunk * S.n / A.nsplit, sh1 = (chunk + 1) * S.n / A.nsplit;
+  for (int sh = sh0; sh < sh1; ++sh) {
     const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
     int n = 0, e = (int)A.nnodes, sp = 0;
     uint64_t M = all;
@@ -220,9 +215,10 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
       const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
       const double d0 = y[0] - (c3[0] + s[0]), d1 = y[1] - (c3[1] + s[1]),
                    d2 = y[2] - (c3[2] + s[2]);
-      const double dist = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-      // epsilon(0.): the node is left alone (avoids 1/0, tree.f90:441)
-      const bool live = mine && !(dist < 2.220446049250313e-16);
+      const double dsq = (d0 * d0 + d1 * d1) + d2 * d2;
+      // dist < epsilon(0.): the node is left alone (avoids 1/0,
+      // tree.f90:441); sqrt(dsq) < 2^-52 is exactly dsq < 2^-104
+      const bool live = mine && !(dsq < 0x1p-104);
       bool taken;
       if (cnt == 1) {
         taken = mine;
@@ -242,39 +238,27 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
           const double ub[3] = {f * (R1[1] * R2[2] - R1[2] * R2[1]),
                                 f * (R1[2] * R2[0] - R1[0] * R2[2]),
                                 f * (R1[0] * R2[1] - R1[1] * R2[0])};
-          for (int k = 0; k < 3; ++k)
-            u[k] = u[k] + (A.ref_scale ? (ub[k] / 4.) / kPi : ((ub[k] * A.qc) / 4.) / kPi);
+          for (int k = 0; k < 3; ++k) ul[k] = ul[k] + ub[k];
         }
       } else {
-        const double theta = A.geo[n].x / dist;
-        const bool cell = live && theta < A.theta;
+        // width/dist < theta (tree.f90:443) as dsq >= the node's exact
+        // threshold; the root (and any node without one) takes the test
+        const double T = A.tplain[n];
+        bool acc;
+        if (uni((int32_t)(__double_as_longlong(T) >> 32)) >= 0) {
+          acc = dsq >= T;
+        } else {
+          acc = A.geo[n].x / sqrt(dsq) < A.theta;
+        }
+        const bool cell = live && acc;
         taken = mine && (!live || cell);
-        if (cell) {
-          const double *mp = A.mp + 12 * (int64_t)n;
-          const double *circ = mp;
-          const double *C = mp + 3;  // C[3*i + k] = Cij(i+1, k+1)
+        if (__ballot(cell)) {
+          // the multipole regrouped (as in the velocity walk): within a few
+          // ulp of the reference's k, l, m loops (tree.f90:466-478)
           const double v[3] = {d0, d1, d2};
-          double ub[3] = {circ[1] * v[2] - circ[2] * v[1], circ[2] * v[0] - circ[0] * v[2],
-                          circ[0] * v[1] - circ[1] * v[0]};
-          const double d3 = (dist * dist) * dist;
-          const double d5 = (((dist * dist) * dist) * dist) * dist;
-          for (int k = 0; k < 3; ++k) ub[k] = ub[k] / d3;
-          // the reference's k, l, m loops (tree.f90:466-478) with the terms
-          // of a zero Levi-Civita symbol left out: each adds (0 * finite) / d
-          // = 0 to its sum, which leaves the sum as it is, so the remaining
-          // terms meet every sum in the reference's order
-          for (int k = 0; k < 3; ++k) {
-            for (int l = 0; l < 3; ++l) {
-              if (k == l) continue;
-              const int m = 3 - k - l;
-              const double ee = ((m + 1) % 3 == k) ? 1. : -1.;
-              const double dcl = (v[0] * C[l] + v[1] * C[3 + l]) + v[2] * C[6 + l];
-              ub[m] = ub[m] - (ee * C[3 * k + l]) / d3;
-              ub[m] = ub[m] + (((3. * ee) * v[k]) * dcl) / d5;
-            }
-          }
-          for (int k = 0; k < 3; ++k)
-            u[k] = A.ref_scale ? u[k] + ub[k] : u[k] + ((ub[k] * A.qc) / 4.) / kPi;
+          double ub[3];
+          cell64(cell, v, dsq, A.mp + 12 * (int64_t)n, ub);
+          for (int k = 0; k < 3; ++k) uc[k] = uc[k] + ub[k];
         }
       }
       const uint64_t O = M & ~__ballot(taken);
@@ -299,8 +283,27 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
       }
     }
   }
-  if (act)
+  if (!act) return;
+  double u[3];
+  for (int k = 0; k < 3; ++k)
+    u[k] = A.ref_scale ? (ul[k] / 4.) / kPi + uc[k] : ((ul[k] + uc[k]) * A.qc / 4.) / kPi;
+  if (A.nsplit == 1) {
     for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
+  } else {
+    double *pp = A.part + 3 * ((int64_t)q * A.nsplit + chunk);
+    for (int k = 0; k < 3; ++k) pp[k] = u[k];
+  }
+}
+
+// u of each probe: its chunks' partial sums in chunk (= shell) order
+__global__ void __launch_bounds__(kBlock) k_probe_join(ProbeArgs A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.nprobe) return;
+  const double *pp = A.part + 3 * (int64_t)q * A.nsplit;
+  double u[3] = {pp[0], pp[1], pp[2]};
+  for (int c = 1; c < A.nsplit; ++c)
+    for (int k = 0; k < 3; ++k) u[k] = u[k] + pp[3 * c + k];
+  for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
 }
 
 }  // namespace
@@ -325,20 +328,26 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
   A.rec = c.tree.rec;
   A.geo = c.tree.geo;
   A.count = c.tree.count;
+  A.tplain = c.tree.tplain;
   A.mp = c.tree.mp;
+  A.nsplit = 1;
+  A.part = nullptr;
   const Shells S = make_shells(c.p);
   const dim3 g((nprobe + kBlock - 1) / kBlock);
   if (tree) {
-    // probes in Morton order: 2 x nprobe keys, then 2 x nprobe indices
-    uint64_t *key = reinterpret_cast<uint64_t *>(d_work), *key_s = key + nprobe;
+    if (!c.tree.plain_valid) {
+      const hipError_t e = launch_plain_thresholds(c);
+      if (e != hipSuccess) return e;
+    }
+    // probes in Hilbert-curve order: 2 x nprobe keys, then 2 x nprobe indices
+    uint32_t *key = reinterpret_cast<uint32_t *>(d_work), *key_s = key + nprobe;
     int32_t *idx = reinterpret_cast<int32_t *>(key_s + nprobe), *idx_s = idx + nprobe;
-    hipLaunchKernelGGL(k_probe_keys, g, dim3(kBlock), 0, c.stream, nprobe, d_y, c.p.box_size,
-                       key, idx);
-    hipError_t err = hipGetLastError();
+    hipError_t err = launch_hilbert_keys(c.stream, nprobe, c.p.box_size, d_y, d_y + 1, d_y + 2,
+                                         3, nullptr, key, idx);
     if (err != hipSuccess) return err;
     size_t need = 0;
-    err = rocprim::radix_sort_pairs(nullptr, need, key, key_s, idx, idx_s, nprobe, 0, 63,
-                                    c.stream);
+    err = rocprim::radix_sort_pairs<SortConfig>(nullptr, need, key, key_s, idx, idx_s, nprobe, 0,
+                                                21, c.stream);
     if (err != hipSuccess) return err;
     if (need > c.sort_tmp_bytes) {
       if ((err = hipStreamSynchronize(c.stream)) != hipSuccess) return err;
@@ -349,10 +358,35 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
       c.sort_tmp_bytes = need;
     }
     size_t have = c.sort_tmp_bytes;
-    err = rocprim::radix_sort_pairs(c.sort_tmp, have, key, key_s, idx, idx_s, nprobe, 0, 63,
-                                    c.stream);
+    err = rocprim::radix_sort_pairs<SortConfig>(c.sort_tmp, have, key, key_s, idx, idx_s, nprobe,
+                                                0, 21, c.stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_probe_tree, g, dim3(kBlock), 0, c.stream, A, S, idx_s);
+    // a few probes cannot fill the GPU one wavefront per 64 of them: their
+    // shells are then split over up to S.n wavefronts (about 8 per SIMD in
+    // all), each with its own partial sums
+    const int64_t groups = (nprobe + kWave - 1) / kWave;
+    const int64_t want = 8 * 1024;
+    A.nsplit = (int)std::min<int64_t>(S.n, std::max<int64_t>(1, (want + groups - 1) / groups));
+    A.part = nullptr;
+    if (A.nsplit > 1) {
+      const size_t need_part = (size_t)3 * nprobe * A.nsplit;
+      if (need_part > c.probe_part_cap) {
+        if ((err = hipStreamSynchronize(c.stream)) != hipSuccess) return err;
+        (void)hipFree(c.probe_part);
+        c.probe_part = nullptr;
+        c.probe_part_cap = 0;
+        if ((err = hipMalloc(&c.probe_part, need_part * sizeof(double))) != hipSuccess) return err;
+        c.probe_part_cap = need_part;
+      }
+      A.part = c.probe_part;
+    }
+    const int64_t waves = groups * A.nsplit;
+    const dim3 gt((unsigned)((waves + kBlock / kWave - 1) / (kBlock / kWave)));
+    hipLaunchKernelGGL(k_probe_tree, gt, dim3(kBlock), 0, c.stream, A, S, idx_s);
+    if (A.nsplit > 1) {
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      hipLaunchKernelGGL(k_probe_join, g, dim3(kBlock), 0, c.stream, A);
+    }
   }
   else if (d_g)
     hipLaunchKernelGGL(k_probe_bs<true>, g, dim3(kBlock), 0, c.stream, A, S);

@@ -350,6 +350,21 @@ __device__ double accept_threshold(double w, double g, double theta, int extra) 
   return bitsd(hi);
 }
 
+// the probe walk's plain test (tree_walk_general, tree.f90:437-444) as a
+// threshold on d2; the root takes the exact test
+__global__ void __launch_bounds__(kBlock) k_plain_thresholds(int64_t nnodes,
+                                                             const double2 *__restrict__ geo,
+                                                             const int32_t *__restrict__ count,
+                                                             double theta,
+                                                             double *__restrict__ tplain) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nnodes) return;
+  double T = 0.;  // a one-point node is never tested
+  if (node == 0) T = -1.;
+  else if (count[node] > 1) T = accept_threshold(geo[node].x, 0., theta, 0);
+  tplain[node] = T;
+}
+
 __device__ inline void finalize_node(const MomArgs &A, int64_t node, int n, double s[15],
                                      bool is_root) {
   const double cx = s[0] / n, cy = s[1] / n, cz = s[2] / n;
@@ -501,24 +516,32 @@ __global__ void __launch_bounds__(kBlock) k_moments_big(MomArgs A) {
   }
 }
 
-// 21-bit Hilbert-curve index of each sorted position on a 128^3 grid of the
-// box (finer grids group the 1M-point workload no better; ties keep the
-// Morton order of the sorted positions) (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
-// 381 (2004): axes to transposed index, then bit interleave); the index
-// itself is the sort's value.  Only the walk's grouping of targets into
-// wavefronts uses this order: each target's own walk and sums do not
-// depend on it.
-__global__ void __launch_bounds__(kBlock) k_hilbert_keys(int nact, double L,
+// 21-bit Hilbert-curve index of each position on a 128^3 grid of the box
+// (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381
+// (2004): axes to transposed index, then bit interleave); slots with
+// infront = 0 (when inf is given) get 2^21 and sort last.  The index itself
+// is the sort's value.  Only the grouping of targets into wavefronts uses
+// this order (finer grids group the 1M-point workload no better; ties keep
+// the input order): each target's own walk and results do not depend on it.
+__global__ void __launch_bounds__(kBlock) k_hilbert_keys(int n, double L,
                                                          const double *__restrict__ xs,
                                                          const double *__restrict__ ys,
                                                          const double *__restrict__ zs,
+                                                         int stride,
+                                                         const int32_t *__restrict__ inf,
                                                          uint32_t *__restrict__ key,
                                                          int32_t *__restrict__ idx) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nact) return;
+  if (s >= n) return;
+  idx[s] = s;
+  if (inf && inf[s] == 0) {
+    key[s] = 1u << 21;
+    return;
+  }
   constexpr int kBits = 7;
   const double scale = (double)(1 << kBits) / L;
-  const double p[3] = {xs[s], ys[s], zs[s]};
+  const int64_t o = (int64_t)s * stride;
+  const double p[3] = {xs[o], ys[o], zs[o]};
   uint32_t X[3];
   for (int k = 0; k < 3; ++k) {
     const double q = (p[k] + L / 2.) * scale;
@@ -547,7 +570,6 @@ __global__ void __launch_bounds__(kBlock) k_hilbert_keys(int nact, double L,
   for (int b = kBits - 1; b >= 0; --b)
     for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
   key[s] = h;
-  idx[s] = s;
 }
 
 hipError_t grow_tree(Tree &t, int64_t need) {
@@ -555,7 +577,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   int64_t cap = need + need / 4 + 1024;
   (void)hipFree(t.rec); (void)hipFree(t.meta); (void)hipFree(t.mp); (void)hipFree(t.mp32); (void)hipFree(t.geo);
   (void)hipFree(t.first); (void)hipFree(t.count); (void)hipFree(t.level); (void)hipFree(t.digit); (void)hipFree(t.corner);
-  (void)hipFree(t.linf);
+  (void)hipFree(t.linf); (void)hipFree(t.tplain);
   t = Tree{};
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&t.rec, cap * sizeof(NodeRec)))) return e;
@@ -569,6 +591,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   if ((e = hipMalloc(&t.digit, cap * sizeof(int8_t)))) return e;
   if ((e = hipMalloc(&t.corner, cap * 3 * sizeof(double)))) return e;
   if ((e = hipMalloc(&t.linf, cap * sizeof(int32_t)))) return e;
+  if ((e = hipMalloc(&t.tplain, cap * sizeof(double)))) return e;
   t.cap = cap;
   return hipSuccess;
 }
@@ -580,6 +603,51 @@ hipError_t grow_tree(Tree &t, int64_t need) {
     hipError_t e_ = (expr);                \
     if (e_ != hipSuccess) return QV_ERR_HIP; \
   } while (0)
+
+hipError_t launch_hilbert_keys(hipStream_t st, int n, double L, const double *x, const double *y,
+                               const double *z, int stride, const int32_t *inf, uint32_t *key,
+                               int32_t *idx) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hilbert_keys, dim3(grid_of(n)), dim3(kBlock), 0, st, n, L, x, y, z, stride,
+                     inf, key, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain_thresholds(Ctx &c) {
+  const int64_t nn = c.tree.nnodes;
+  if (nn <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plain_thresholds, dim3((unsigned)((nn + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, c.stream, nn, c.tree.geo, c.tree.count, c.p.tree_theta,
+                     c.tree.tplain);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) c.tree.plain_valid = true;
+  return e;
+}
+
+int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order) {
+  if (n <= 0) return QV_OK;
+  // the key and slot buffers of the build are free outside tree_build's
+  // steps 1-4: keys in key0 (2 cap 32-bit words), indices in slot
+  uint32_t *hk = reinterpret_cast<uint32_t *>(c.key0);
+  uint32_t *hk_s = hk + c.cap;
+  hipStream_t st = c.stream;
+  QV_TRY(launch_hilbert_keys(st, n, c.p.box_size, pos.x, pos.y, pos.z, 1, inf, hk, c.slot));
+  const unsigned end_bit = inf ? 22 : 21;
+  size_t need = 0;
+  QV_TRY(rocprim::radix_sort_pairs<SortConfig>(nullptr, need, hk, hk_s, c.slot, order, n, 0,
+                                               end_bit, st));
+  if (need > c.sort_tmp_bytes) {
+    (void)hipFree(c.sort_tmp);
+    c.sort_tmp = nullptr;
+    c.sort_tmp_bytes = 0;
+    QV_TRY(hipMalloc(&c.sort_tmp, need));
+    c.sort_tmp_bytes = need;
+  }
+  size_t have = c.sort_tmp_bytes;
+  QV_TRY(rocprim::radix_sort_pairs<SortConfig>(c.sort_tmp, have, hk, hk_s, c.slot, order, n, 0,
+                                               end_bit, st));
+  return QV_OK;
+}
 
 int tree_build(Ctx &c) {
   const int n = c.pcount;
@@ -613,28 +681,8 @@ int tree_build(Ctx &c) {
                      c.key1, key1_s, c.x.x, c.x.y, c.x.z, c.gi.x, c.gi.y, c.gi.z, c.xs.x, c.xs.y,
                      c.xs.z, c.gs.x, c.gs.y, c.gs.z);
   QV_TRY(hipGetLastError());
-  // 4b. the walk's order of the sorted targets (Hilbert curve); the
-  // unsorted key and slot buffers are free from here on
-  if (nact > 0) {
-    uint32_t *hk = reinterpret_cast<uint32_t *>(c.key0);
-    uint32_t *hk_s = hk + c.cap;  // key0 holds 2 cap 32-bit words
-    hipLaunchKernelGGL(k_hilbert_keys, dim3(grid_of(nact)), dim3(kBlock), 0, st, nact,
-                       c.p.box_size, c.xs.x, c.xs.y, c.xs.z, hk, c.slot);
-    QV_TRY(hipGetLastError());
-    size_t hneed = 0;
-    QV_TRY(rocprim::radix_sort_pairs<SortConfig>(nullptr, hneed, hk, hk_s, c.slot, c.walk_order,
-                                                 nact, 0, 21, st));
-    if (hneed > c.sort_tmp_bytes) {
-      (void)hipFree(c.sort_tmp);
-      c.sort_tmp = nullptr;
-      c.sort_tmp_bytes = 0;
-      QV_TRY(hipMalloc(&c.sort_tmp, hneed));
-      c.sort_tmp_bytes = hneed;
-    }
-    size_t hhave = c.sort_tmp_bytes;
-    QV_TRY(rocprim::radix_sort_pairs<SortConfig>(c.sort_tmp, hhave, hk, hk_s, c.slot, c.walk_order,
-                                                 nact, 0, 21, st));
-  }
+  // 4b. the walk's order of the sorted targets (Hilbert curve)
+  if (int rc = hilbert_order(c, nact, c.xs, nullptr, c.walk_order)) return rc;
   // 5. nodes per sorted position, prefix sum -> DFS preorder offsets
   QV_TRY(hipMemsetAsync(c.nnode_of + nact, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_count_nodes, dim3(grid_of(nact)), dim3(kBlock), 0, st, nact, n, c.key0_s,
@@ -665,6 +713,7 @@ int tree_build(Ctx &c) {
   if (nnodes < 1 || nnodes >= (int64_t)INT32_MAX) return QV_ERR_ARG;
   QV_TRY(grow_tree(c.tree, nnodes));
   c.tree.nnodes = nnodes;
+  c.tree.plain_valid = false;
   // 6. node records
   NodeArgs NA;
   NA.nact = nact;

@@ -90,6 +90,49 @@ def test_tree_probe_vs_oracle(theta):
     assert _rel(ur, refr) <= 1e-12
 
 
+def test_tree_probe_follows_theta_of_set_params():
+    """The probe walk's acceptance thresholds are per tree and per theta: a
+    new tree_theta through set_params applies to the next probe call on the
+    same tree."""
+    from oracle import port
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, target_points=5_000, seed=57)
+    kw = dict(velocity="Tree", boundary="periodic", box_size=0.1, corea=1.3e-8)
+    rng = np.random.Generator(np.random.PCG64(58))
+    y = rng.uniform(-0.05, 0.05, size=(500, 3))
+    with gpu.Context(gpu.params(tree_theta=0.4, **kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u4 = c.probe_velocity(y)
+        c.set_params(gpu.params(tree_theta=0.6, **kw))
+        u6 = c.probe_velocity(y)
+    for th, u in ((0.4, u4), (0.6, u6)):
+        p = port.make_params(tree_theta=th, **kw)
+        gi, _, _, _ = port.ghostp(p, fil)
+        ref = port.probe_velocity(p, fil, fil, gi, y, ref_scaling=False, nthreads=16)
+        assert _rel(u, ref) <= 1e-12
+
+
+def test_tree_probe_split_and_unsplit_agree():
+    """Few probes split their shells over several wavefronts (partial sums
+    added in shell order), many do not: the same probes agree both ways."""
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, target_points=20_000, seed=55)
+    kw = dict(velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.1,
+              corea=1.3e-8)
+    rng = np.random.Generator(np.random.PCG64(56))
+    y = rng.uniform(-0.05, 0.05, size=(600_000, 3))
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u_all = c.probe_velocity(y)
+        u_few = c.probe_velocity(y[:3000])
+    _say("split vs unsplit", _rel(u_few, u_all[:3000]))
+    assert _rel(u_few, u_all[:3000]) <= 1e-13
+
+
 def test_gradient_vs_differences_and_divergence():
     from oracle import port
     from qvort_amd import geometry as G
