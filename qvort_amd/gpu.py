@@ -279,15 +279,24 @@ class Context:
         _check(lib().qv_tree_dump(self._h, _dp(out), nn.value, C.byref(nn)), "qv_tree_dump")
         return out
 
-    def closest(self, closest_in=None, closestd_in=None):
+    def closest(self, closest_in=None, closestd_in=None, inplace: bool = False):
         """Neighbour search of the reconnection step (pclose_tree with
         tree_theta > 0, else pclose) on the current slots: (closest, closestd),
-        from the f(:)%closest / %closestd values given (zeros if None)."""
+        from the f(:)%closest / %closestd values given (zeros if None).  With
+        inplace=True the given arrays (C-contiguous int32 and float64, one
+        entry per slot, e.g. pinned host memory) receive the results."""
         n = self.pcount
-        cl = np.zeros(n, dtype=np.int32) if closest_in is None else \
-            np.array(closest_in, dtype=np.int32, order="C").copy()
-        cd = np.zeros(n) if closestd_in is None else \
-            np.array(closestd_in, dtype=np.float64, order="C").copy()
+        if inplace:
+            cl, cd = closest_in, closestd_in
+            if not (isinstance(cl, np.ndarray) and isinstance(cd, np.ndarray)
+                    and cl.dtype == np.int32 and cd.dtype == np.float64
+                    and cl.flags.c_contiguous and cd.flags.c_contiguous):
+                raise ValueError("inplace needs C-contiguous int32 and float64 arrays")
+        else:
+            cl = np.zeros(n, dtype=np.int32) if closest_in is None else \
+                np.array(closest_in, dtype=np.int32, order="C")
+            cd = np.zeros(n) if closestd_in is None else \
+                np.array(closestd_in, dtype=np.float64, order="C")
         if cl.shape != (n,) or cd.shape != (n,):
             raise ValueError("closest/closestd must have one entry per slot")
         _check(lib().qv_closest(self._h, _ip(cl), _dp(cd)), "qv_closest")

@@ -138,9 +138,7 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
     pinsert(p, st, counts)
     if search:
         ctx.upload(st.x, st.infront, st.behind)
-        cl, cd = ctx.closest(st.closest, st.closestd)
-        st.closest[:] = cl
-        st.closestd[:] = cd
+        ctx.closest(st.closest, st.closestd, inplace=True)
         precon(p, st, counts)
         premove(p, st, counts)
     ctx.upload(st.x, st.infront, st.behind)
