@@ -20,7 +20,8 @@
 //                  recursive walk.  Fewer than about half a million probes
 //                  cannot fill the GPU that way: their shells are split over
 //                  several wavefronts per group of 64 probes, whose partial
-//                  sums are added in shell order after the walk.  The
+//                  sums are added in shell order after the walk.  Probes
+//                  much sparser than the points walk one per lane.  The
 //                  reference adds leaves as
 //                  u_bs/4/pi (no Gamma) and cells without Gamma/(4 pi); by
 //                  default both get Gamma/(4 pi) here, and
@@ -171,6 +172,9 @@ __global__ void __launch_bounds__(kBlock) k_probe_bs(ProbeArgs A, Shells S) {
     for (int k = 0; k < 9; ++k) A.g[9 * q + k] = G[k];
 }
 
+// probes fewer than points / kLaneWalkRatio take one walk per lane
+constexpr int64_t kLaneWalkRatio = 16;
+
 struct alignas(16) ProbeEntry {
   int32_t node, end;
   uint64_t mask;
@@ -295,6 +299,79 @@ __global__ void __launch_bounds__(kBlock) k_probe_tree(ProbeArgs A, Shells S,
   }
 }
 
+// one walk per lane, for probes too scattered for a wavefront to share one:
+// stackless, since an opened node continues at n + 1 (its first stored
+// child) and any other node at its skip pointer (the next node of the
+// preorder outside its subtree), so every lane meets its own nodes in the
+// DFS order of tree_walk_general.  Shell chunks and output as k_probe_tree.
+__global__ void __launch_bounds__(kBlock) k_probe_lane(ProbeArgs A, Shells S,
+                                                       const int32_t *__restrict__ order) {
+  const int gw = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int chunk = gw % A.nsplit;
+  const int qs = (gw / A.nsplit) * kWave + threadIdx.x % kWave;
+  const bool act = qs < A.nprobe;
+  const int q = act ? order[qs] : 0;
+  double y[3] = {0., 0., 0.};
+  if (act)
+    for (int k = 0; k < 3; ++k) y[k] = A.py[3 * q + k];
+  const int nn = (int)A.nnodes;
+  double ul[3] = {0., 0., 0.}, uc[3] = {0., 0., 0.};
+  const int sh0 = chunk * S.n / A.nsplit, sh1 = (chunk + 1) * S.n / A.nsplit;
+  for (int sh = sh0; sh < sh1; ++sh) {
+    const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
+    int n = act ? 0 : nn;
+    while (n < nn) {
+      const NodeRec r = A.rec[n];
+      const double c3[3] = {r.cx, r.cy, r.cz};
+      const double d0 = y[0] - (c3[0] + s[0]), d1 = y[1] - (c3[1] + s[1]),
+                   d2 = y[2] - (c3[2] + s[2]);
+      const double dsq = (d0 * d0 + d1 * d1) + d2 * d2;
+      // dist < epsilon(0.) leaves the node alone (tree.f90:441)
+      const bool live = !(dsq < 0x1p-104);
+      bool open = false;
+      if (A.count[n] == 1) {
+        if (live) {
+          double R1[3], R2[3];
+          for (int k = 0; k < 3; ++k) {
+            const double circ = r.g[k] - c3[k];
+            R1[k] = (c3[k] + s[k]) - y[k];
+            R2[k] = ((c3[k] + circ) + s[k]) - y[k];
+          }
+          const double a = sqrt((R1[0] * R1[0] + R1[1] * R1[1]) + R1[2] * R1[2]);
+          const double b = sqrt((R2[0] * R2[0] + R2[1] * R2[1]) + R2[2] * R2[2]);
+          const double cc =
+              (a * b) * ((a * b) + ((R1[0] * R2[0] + R1[1] * R2[1]) + R1[2] * R2[2]));
+          const double f = (a + b) / cc;
+          ul[0] = ul[0] + f * (R1[1] * R2[2] - R1[2] * R2[1]);
+          ul[1] = ul[1] + f * (R1[2] * R2[0] - R1[0] * R2[2]);
+          ul[2] = ul[2] + f * (R1[0] * R2[1] - R1[1] * R2[0]);
+        }
+      } else {
+        const double T = A.tplain[n];
+        const bool acc = T >= 0. ? dsq >= T : A.geo[n].x / sqrt(dsq) < A.theta;
+        if (live && acc) {
+          const double v[3] = {d0, d1, d2};
+          double ub[3];
+          cell64(true, v, dsq, A.mp + 12 * (int64_t)n, ub);
+          for (int k = 0; k < 3; ++k) uc[k] = uc[k] + ub[k];
+        }
+        open = live && !acc;
+      }
+      n = open ? n + 1 : r.skip;
+    }
+  }
+  if (!act) return;
+  double u[3];
+  for (int k = 0; k < 3; ++k)
+    u[k] = A.ref_scale ? (ul[k] / 4.) / kPi + uc[k] : ((ul[k] + uc[k]) * A.qc / 4.) / kPi;
+  if (A.nsplit == 1) {
+    for (int k = 0; k < 3; ++k) A.u[3 * q + k] = u[k];
+  } else {
+    double *pp = A.part + 3 * ((int64_t)q * A.nsplit + chunk);
+    for (int k = 0; k < 3; ++k) pp[k] = u[k];
+  }
+}
+
 // u of each probe: its chunks' partial sums in chunk (= shell) order
 __global__ void __launch_bounds__(kBlock) k_probe_join(ProbeArgs A) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,7 +459,12 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
     }
     const int64_t waves = groups * A.nsplit;
     const dim3 gt((unsigned)((waves + kBlock / kWave - 1) / (kBlock / kWave)));
-    hipLaunchKernelGGL(k_probe_tree, gt, dim3(kBlock), 0, c.stream, A, S, idx_s);
+    // a wavefront shares one walk among 64 probes adjacent on the curve when
+    // they are about as dense as the points; scattered probes walk alone
+    if ((int64_t)nprobe * kLaneWalkRatio < (int64_t)c.nact)
+      hipLaunchKernelGGL(k_probe_lane, gt, dim3(kBlock), 0, c.stream, A, S, idx_s);
+    else
+      hipLaunchKernelGGL(k_probe_tree, gt, dim3(kBlock), 0, c.stream, A, S, idx_s);
     if (A.nsplit > 1) {
       if ((err = hipGetLastError()) != hipSuccess) return err;
       hipLaunchKernelGGL(k_probe_join, g, dim3(kBlock), 0, c.stream, A);

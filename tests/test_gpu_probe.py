@@ -114,6 +114,26 @@ def test_tree_probe_follows_theta_of_set_params():
         assert _rel(u, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("nprobe", [1, 63, 65, 130])
+def test_tree_probe_odd_counts_vs_oracle(nprobe):
+    """Partial groups of 64 and shell chunks of unequal length (27 shells
+    over up to 27 wavefronts per group)."""
+    from oracle import port
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, target_points=3_000, seed=59)
+    kw = dict(velocity="Tree", tree_theta=0.5, boundary="openx", box_size=0.1, corea=1.3e-8)
+    y = np.random.Generator(np.random.PCG64(60)).uniform(-0.06, 0.06, size=(nprobe, 3))
+    p = port.make_params(**kw)
+    with gpu.Context(gpu.params(**kw)) as c:
+        c.upload_filaments(fil)
+        c.velocity()
+        u = c.probe_velocity(y, reference_scale=True)
+    gi, _, _, _ = port.ghostp(p, fil)
+    ref = port.probe_velocity(p, fil, fil, gi, y, ref_scaling=True, nthreads=16)
+    assert _rel(u, ref) <= 1e-12
+
+
 def test_tree_probe_split_and_unsplit_agree():
     """Few probes split their shells over several wavefronts (partial sums
     added in shell order), many do not: the same probes agree both ways."""
