@@ -190,9 +190,10 @@ def run_next_rows(fil, theta: float) -> dict:
     """The calls next to the velocity stage on the same workload, wall-clock
     per call from Python (host-device copies included): the reconnection
     step's neighbour search on the stage's tree (pclose_tree), the velocity
-    at 100,000 random probe points by tree_walk_general's rules
-    (mesh_velocity), and one topology step (pinsert on the host, the search,
-    precon and premove on the host, state down and back up)."""
+    at 1,000 and 100,000 random probe points by tree_walk_general's rules
+    (mesh_velocity), one topology step (pinsert on the host, the search,
+    precon and premove on the host, state down and back up), and one whole
+    step with it (velocity stage, AB update, topology step, wrap)."""
     from qvort_amd import gpu, line
     p = gpu.params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
                    delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6, precision="fp32c")
@@ -208,6 +209,9 @@ def run_next_rows(fil, theta: float) -> dict:
         out["closest_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
         c.probe_velocity(y[:1000])
         t0 = time.perf_counter()
+        c.probe_velocity(y[:1000])
+        out["probe_tree_1k_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        t0 = time.perf_counter()
         c.probe_velocity(y)
         out["probe_tree_100k_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
         st = line.LineState.from_arrays(capacity=2 * fil.pcount + 64, x=fil.x,
@@ -218,6 +222,15 @@ def run_next_rows(fil, theta: float) -> dict:
         line.topology_step(c, p, st, cnt, search=True)
         out["topology_step_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
         out["topology_counts"] = line.counts_dict(cnt)
+        # one whole step as run.f90 orders it: velocity stage, AB update,
+        # line operations with the search, wrap
+        c.enforce_periodic()
+        t0 = time.perf_counter()
+        c.velocity_only()
+        c.ab_step()
+        line.topology_step(c, p, st, cnt, search=True)
+        c.enforce_periodic()
+        out["step_with_topology_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
     return out
 
 
