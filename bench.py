@@ -93,6 +93,29 @@ def cpu_baseline(fil, theta: float, n_sample: int):
                         f"targets ({walk_s:.2f} s), stage time extrapolated to all targets"))
 
 
+def error_vs_direct(fil, theta: float, u32, u64, n_sample: int = 200) -> dict:
+    """Part of the CPU leg: the stage's velocities against the FP64 direct sum
+    on a strided sample of targets -- the theta -> 0+ limit of the tree (every
+    leaf with the walk's exclusions, shells composed as calc_velocity does),
+    summed by the C oracle on this host.  This is the truncation error of the
+    reference's multipole acceptance (identical in the reference) plus, for
+    FP32C, its own rounding."""
+    from oracle import port
+    p = port.make_params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
+                         corea=1.3e-8)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    act = np.flatnonzero(fil.infront)
+    sample = act[::max(1, act.size // n_sample)][:n_sample].astype(np.int32)
+    ref, _ = port.leaf_sum_targets(p, fil, sample, nthreads=nthreads)
+    nr = np.linalg.norm(ref, axis=1)
+    out = {"vs": "fp64 direct sum (theta -> 0+ leaf sum, tree exclusions), C oracle",
+           "sample": int(sample.size)}
+    for name, u in (("fp32c", u32), ("fp64", u64)):
+        e = np.linalg.norm(u[sample] - ref, axis=1) / nr
+        out[name] = {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e ** 2)))}
+    return out
+
+
 FLOP_PEAK_TFS = {"fp32c": 157.3, "fp64": 78.6}  # MI355X published fp32 / fp64
 
 
@@ -329,6 +352,7 @@ def main():
         print(f"next rows: {out['next_rows']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fil, args.theta, args.cpu_sample)
+        out["error_vs_direct"] = error_vs_direct(fil, args.theta, r32["u"], r64["u"])
     print(json.dumps(out), flush=True)
 
 
