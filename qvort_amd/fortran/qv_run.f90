@@ -90,12 +90,14 @@ program qv_run
     integer(c_int32_t) :: closest
     real(c_double) :: closestd
     real(c_double) :: closestd_loop
-    logical(4) :: pinnedi
-    logical(4) :: pinnedb
+    integer(c_int32_t) :: pinnedi  ! logical(4) in type qvort: its 4 bytes (0 = .false.)
+    integer(c_int32_t) :: pinnedb
     real(c_double) :: delta
     real(c_double) :: t_recon(2)
   end type
   integer :: nstart = 1
+  ! bytes of one f(:) element in data/var.dat
+  integer(c_int64_t), parameter :: kRecordBytes = 460
   ! state
   type(qv_params) :: p
   type(c_ptr) :: ctx
@@ -171,8 +173,15 @@ program qv_run
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
     t = t + dt
     if (mod(itime, shots) == 0) then
-      call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
-                    'qv_download')
+      ! after the line operations the host holds the step's velocities (an
+      ! upload leaves none on the device); positions and history come back
+      if (line_ops) then
+        call qv_check(qv_download(ctx, c_loc(x), c_null_ptr, c_loc(u1), c_loc(u2), c_loc(u3)), &
+                      'qv_download')
+      else
+        call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
+                      'qv_download')
+      end if
       call data_dump()
       call print_info(itime / shots)
       call printf(itime / shots)
@@ -316,19 +325,104 @@ contains
     end select
   end subroutine
 
+  !> f(:) in the file is its elements' components back to back, 460 bytes
+  !> each (the Fortran runtime transfers a derived type component by
+  !> component, without the alignment padding of the memory layout); these
+  !> copy whole runs of components between the two layouts, where the
+  !> runtime's item-by-item transfer takes microseconds per component
+  subroutine record_runs(rec, nrun, mo, fo, ln)
+    type(qvort_record), target, intent(in) :: rec
+    integer, intent(out) :: nrun
+    integer, intent(out) :: mo(27), fo(27), ln(27)
+    integer(c_intptr_t) :: base, off(27)
+    integer :: sz(27), k, f
+    base = transfer(c_loc(rec), base)
+    off = [transfer(c_loc(rec%x), base), transfer(c_loc(rec%x_interp), base), &
+           transfer(c_loc(rec%u), base), transfer(c_loc(rec%u1), base), &
+           transfer(c_loc(rec%u2), base), transfer(c_loc(rec%u3), base), &
+           transfer(c_loc(rec%acc), base), transfer(c_loc(rec%u_sup), base), &
+           transfer(c_loc(rec%u_mf), base), transfer(c_loc(rec%u_s_LI), base), &
+           transfer(c_loc(rec%u_s_BS), base), transfer(c_loc(rec%ghosti), base), &
+           transfer(c_loc(rec%ghostb), base), transfer(c_loc(rec%ghostii), base), &
+           transfer(c_loc(rec%ghostbb), base), transfer(c_loc(rec%f_mf), base), &
+           transfer(c_loc(rec%stretch), base), transfer(c_loc(rec%curv), base), &
+           transfer(c_loc(rec%infront), base), transfer(c_loc(rec%behind), base), &
+           transfer(c_loc(rec%closest), base), transfer(c_loc(rec%closestd), base), &
+           transfer(c_loc(rec%closestd_loop), base), transfer(c_loc(rec%pinnedi), base), &
+           transfer(c_loc(rec%pinnedb), base), transfer(c_loc(rec%delta), base), &
+           transfer(c_loc(rec%t_recon), base)] - base
+    sz = [(24, k = 1, 16), 8, 8, 4, 4, 4, 8, 8, 4, 4, 8, 16]
+    nrun = 0
+    f = 0
+    do k = 1, 27
+      if (nrun > 0) then
+        if (int(off(k)) - f == mo(nrun) - fo(nrun)) then
+          ln(nrun) = ln(nrun) + sz(k)
+          f = f + sz(k)
+          cycle
+        end if
+      end if
+      nrun = nrun + 1
+      mo(nrun) = int(off(k)); fo(nrun) = f; ln(nrun) = sz(k)
+      f = f + sz(k)
+    end do
+  end subroutine
+
+  subroutine pack_records(rec, bytes)
+    type(qvort_record), target, intent(in) :: rec(:)
+    integer(c_int8_t), allocatable, intent(out) :: bytes(:)
+    integer(c_int8_t), pointer :: mem(:)
+    integer :: nrun, mo(27), fo(27), ln(27), r
+    integer(c_int64_t) :: i, m, f, stride
+    allocate(bytes(size(rec, kind=c_int64_t) * kRecordBytes))
+    if (size(rec) == 0) return
+    stride = storage_size(rec(1)) / 8
+    call record_runs(rec(1), nrun, mo, fo, ln)
+    call c_f_pointer(c_loc(rec), mem, [size(rec, kind=c_int64_t) * stride])
+    do i = 0, size(rec, kind=c_int64_t) - 1
+      m = i * stride
+      f = i * kRecordBytes
+      do r = 1, nrun
+        bytes(f + fo(r) + 1:f + fo(r) + ln(r)) = mem(m + mo(r) + 1:m + mo(r) + ln(r))
+      end do
+    end do
+  end subroutine
+
+  subroutine unpack_records(bytes, rec)
+    integer(c_int8_t), intent(in) :: bytes(:)
+    type(qvort_record), target, intent(inout) :: rec(:)
+    integer(c_int8_t), pointer :: mem(:)
+    integer :: nrun, mo(27), fo(27), ln(27), r
+    integer(c_int64_t) :: i, m, f, stride
+    if (size(rec) == 0) return
+    stride = storage_size(rec(1)) / 8
+    call record_runs(rec(1), nrun, mo, fo, ln)
+    call c_f_pointer(c_loc(rec), mem, [size(rec, kind=c_int64_t) * stride])
+    do i = 0, size(rec, kind=c_int64_t) - 1
+      m = i * stride
+      f = i * kRecordBytes
+      do r = 1, nrun
+        mem(m + mo(r) + 1:m + mo(r) + ln(r)) = bytes(f + fo(r) + 1:f + fo(r) + ln(r))
+      end do
+    end do
+  end subroutine
+
   !> data_restore's reads (initial.f90:506-525); quasi particles and
   !> particles are not part of this driver and must be absent
   subroutine data_restore()
     integer :: fh, dummy_itime, quasi_pcount, part_count, recon_count
-    type(qvort_record), allocatable :: rec(:)
+    type(qvort_record), allocatable, target :: rec(:)
+    integer(c_int8_t), allocatable :: bytes(:)
     open(newunit=fh, file='./data/var.dat', form='unformatted', status='old', action='read')
     read(fh) pcount
     read(fh) recon_count
     counts%recon_count = recon_count
     read(fh) dummy_itime
     read(fh) t
-    allocate(rec(pcount))
-    read(fh) rec
+    allocate(rec(pcount), bytes(int(pcount, c_int64_t) * kRecordBytes))
+    read(fh) bytes
+    call unpack_records(bytes, rec)
+    deallocate(bytes)
     read(fh) quasi_pcount
     if (quasi_pcount > 0) call stop_with('data_restore', 'quasi particles are not supported')
     read(fh) part_count
@@ -357,7 +451,8 @@ contains
   !> data_dump's writes (output.f90:274-283)
   subroutine data_dump()
     integer :: fh
-    type(qvort_record), allocatable :: rec(:)
+    type(qvort_record), allocatable, target :: rec(:)
+    integer(c_int8_t), allocatable :: bytes(:)
     allocate(rec(pcount))
     do i = 1, pcount
       rec(i)%x = x(:, i)
@@ -383,8 +478,8 @@ contains
       rec(i)%closest = 0
       rec(i)%closestd = 0._c_double
       rec(i)%closestd_loop = 0._c_double
-      rec(i)%pinnedi = .false.
-      rec(i)%pinnedb = .false.
+      rec(i)%pinnedi = 0
+      rec(i)%pinnedb = 0
       rec(i)%delta = 0._c_double
       rec(i)%t_recon = 0._c_double
     end do
@@ -393,7 +488,9 @@ contains
     write(fh) int(counts%recon_count, c_int32_t)
     write(fh) itime
     write(fh) t
-    write(fh) rec
+    call pack_records(rec, bytes)
+    write(fh) bytes
+    deallocate(bytes)
     write(fh) 0_c_int32_t
     write(fh) 0_c_int32_t
     close(fh)

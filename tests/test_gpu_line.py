@@ -89,3 +89,11 @@ def test_fortran_driver_with_line_operations(name, tmp_path):
     assert np.array_equal(f["behind"], g["final__behind"])
     act = g["final__infront"] != 0
     assert np.max(np.abs(f["x"] - g["final__x"])[act]) <= 1e-15
+    # u is the step's velocity as the host operations left it (interpolated
+    # for inserted points, zero for removed ones): where the AB history was
+    # not reset by pinsert, u = u1
+    keep = act & np.any(g["final__u1"] != 0, axis=1)
+    assert keep.sum() > 0
+    ref = g["final__u1"][keep]
+    assert np.max(np.linalg.norm(f["u"][keep] - ref, axis=1) / np.linalg.norm(ref, axis=1)) <= 1e-12
+    assert not np.any(f["u"][~act])
