@@ -286,6 +286,16 @@ int qv_line_precon(const qv_params *params, qv_line_state *s, qv_line_counts *c)
  * infront(infront(i)) are closer than 0.998 delta */
 int qv_line_premove(const qv_params *params, qv_line_state *s, qv_line_counts *c);
 
+/* pinned (page-locked) host memory for the caller's f(:) arrays, so that
+ * qv_upload / qv_download / qv_download_ghosts copy at the full PCIe rate
+ * (pageable arrays are staged by the runtime).  No counterpart in the
+ * reference, whose f(:) lives in ordinary Fortran allocatables.
+ * qv_host_alloc: *ptr = `bytes` bytes, or NULL and QV_ERR_HIP (e.g. no
+ * device), in which case ordinary memory serves as well, only slower.
+ * qv_host_free: frees what qv_host_alloc returned; NULL is ignored. */
+int qv_host_alloc(size_t bytes, void **ptr);
+int qv_host_free(void *ptr);
+
 int qv_get_timings(qv_ctx *ctx, qv_timings *t);
 const char *qv_status_string(int status);
 

@@ -452,6 +452,21 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, doubl
   return QV_OK;
 }
 
+int qv_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return QV_ERR_ARG;
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return QV_ERR_HIP;
+  }
+  return QV_OK;
+}
+
+int qv_host_free(void *ptr) {
+  if (!ptr) return QV_OK;
+  return hipHostFree(ptr) == hipSuccess ? QV_OK : QV_ERR_HIP;
+}
+
 int qv_download_ghosts(qv_ctx *ctx, double *gi, double *gb, double *gii, double *gbb) {
   if (!ctx) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);

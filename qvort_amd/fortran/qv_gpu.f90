@@ -63,6 +63,7 @@ module qv_gpu
             qv_velocity, qv_ab_step, qv_rk3_step, qv_enforce_periodic, qv_step, qv_download, &
             qv_tree_dump, qv_closest, qv_probe_velocity, qv_vjp, qv_jvp, qv_get_timings, &
             qv_download_ghosts, qv_line_pinsert, qv_line_precon, qv_line_premove, &
+            qv_host_alloc, qv_host_free, &
             qv_status_string, qv_check
 
   interface
@@ -152,6 +153,16 @@ module qv_gpu
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: v(3, *)
       real(c_double), intent(out) :: jv(3, *)
+    end function
+    !> pinned host memory for the f(:) arrays (QV_ERR_HIP: use ordinary memory)
+    integer(c_int) function qv_host_alloc(bytes, ptr) bind(C, name='qv_host_alloc')
+      import :: c_int, c_size_t, c_ptr
+      integer(c_size_t), value :: bytes
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function qv_host_free(ptr) bind(C, name='qv_host_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
     end function
     !> ghost zones of the last ghostp, each x(3,pcount)-shaped (c_null_ptr skips)
     integer(c_int) function qv_download_ghosts(ctx, gi, gb, gii, gbb) &

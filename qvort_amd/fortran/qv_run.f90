@@ -52,11 +52,16 @@ program qv_run
   real(c_double), parameter :: pi = 3.14159265358979324_c_double
   ! f(:) arrays
   ! arrays hold capacity >= pcount slots; the first pcount are f(:)
-  real(c_double), allocatable, target :: x(:,:), u(:,:), u1(:,:), u2(:,:), u3(:,:)
-  integer(c_int32_t), allocatable, target :: infront(:), behind(:)
+  ! (pinned host memory from qv_host_alloc where the runtime gives it,
+  ! ordinary memory otherwise: host_pinned)
+  real(c_double), pointer, contiguous :: x(:,:) => null(), u(:,:) => null(), &
+    u1(:,:) => null(), u2(:,:) => null(), u3(:,:) => null()
+  integer(c_int32_t), pointer, contiguous :: infront(:) => null(), behind(:) => null()
   ! ghost zones and neighbour-search results, for the line operations
-  real(c_double), allocatable, target :: gi(:,:), gb(:,:), gii(:,:), gbb(:,:), closestd(:)
-  integer(c_int32_t), allocatable, target :: closest(:)
+  real(c_double), pointer, contiguous :: gi(:,:) => null(), gb(:,:) => null(), &
+    gii(:,:) => null(), gbb(:,:) => null(), closestd(:) => null()
+  integer(c_int32_t), pointer, contiguous :: closest(:) => null()
+  logical :: host_pinned = .true., host_decided = .false.
   integer(c_int32_t) :: pcount
   ! run.in
   integer :: nsteps = 1, shots = 1, init_pcount = 100
@@ -523,37 +528,135 @@ contains
     integer :: cap
     cap = pcount
     if (line_ops) cap = 2 * pcount + 64
-    allocate(x(3, cap), u(3, cap), u1(3, cap), u2(3, cap), u3(3, cap))
-    allocate(gi(3, cap), gb(3, cap), gii(3, cap), gbb(3, cap))
-    allocate(infront(cap), behind(cap), closest(cap), closestd(cap))
-    x = 0._c_double; u = 0._c_double; u1 = 0._c_double; u2 = 0._c_double; u3 = 0._c_double
-    gi = 0._c_double; gb = 0._c_double; gii = 0._c_double; gbb = 0._c_double
-    infront = 0; behind = 0; closest = 0; closestd = 0._c_double
+    call new3(x, cap); call new3(u, cap); call new3(u1, cap); call new3(u2, cap)
+    call new3(u3, cap)
+    call new3(gi, cap); call new3(gb, cap); call new3(gii, cap); call new3(gbb, cap)
+    call newi(infront, cap); call newi(behind, cap); call newi(closest, cap)
+    call new1(closestd, cap)
+  end subroutine
+
+  !> pinned host memory for an f(:) array.  The first request decides for
+  !> the run: without pinned memory (no device, say) every array is ordinary
+  !> memory; a later failure is fatal, so that all arrays stay of one kind
+  function host_mem(bytes) result(p)
+    integer(c_size_t), intent(in) :: bytes
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (.not. host_pinned) return
+    if (qv_host_alloc(bytes, p) == QV_OK) then
+      host_decided = .true.
+      return
+    end if
+    if (host_decided) call stop_with('qv_host_alloc', 'out of pinned host memory')
+    host_decided = .true.
+    host_pinned = .false.
+    p = c_null_ptr
+  end function
+
+  !> zeroed (3, cap) / (cap) arrays
+  subroutine new3(a, cap)
+    real(c_double), pointer, contiguous, intent(out) :: a(:,:)
+    integer, intent(in) :: cap
+    type(c_ptr) :: p
+    p = host_mem(int(24, c_size_t) * cap)
+    if (c_associated(p)) then
+      call c_f_pointer(p, a, [3, cap])
+    else
+      allocate(a(3, cap))
+    end if
+    a = 0._c_double
+  end subroutine
+
+  subroutine new1(a, cap)
+    real(c_double), pointer, contiguous, intent(out) :: a(:)
+    integer, intent(in) :: cap
+    type(c_ptr) :: p
+    p = host_mem(int(8, c_size_t) * cap)
+    if (c_associated(p)) then
+      call c_f_pointer(p, a, [cap])
+    else
+      allocate(a(cap))
+    end if
+    a = 0._c_double
+  end subroutine
+
+  subroutine newi(a, cap)
+    integer(c_int32_t), pointer, contiguous, intent(out) :: a(:)
+    integer, intent(in) :: cap
+    type(c_ptr) :: p
+    p = host_mem(int(4, c_size_t) * cap)
+    if (c_associated(p)) then
+      call c_f_pointer(p, a, [cap])
+    else
+      allocate(a(cap))
+    end if
+    a = 0
+  end subroutine
+
+  subroutine free3(a)
+    real(c_double), pointer, contiguous, intent(inout) :: a(:,:)
+    if (host_pinned) then
+      call qv_check(qv_host_free(c_loc(a)), 'qv_host_free')
+    else
+      deallocate(a)
+    end if
+    nullify(a)
+  end subroutine
+
+  subroutine free1(a)
+    real(c_double), pointer, contiguous, intent(inout) :: a(:)
+    if (host_pinned) then
+      call qv_check(qv_host_free(c_loc(a)), 'qv_host_free')
+    else
+      deallocate(a)
+    end if
+    nullify(a)
+  end subroutine
+
+  subroutine freei(a)
+    integer(c_int32_t), pointer, contiguous, intent(inout) :: a(:)
+    if (host_pinned) then
+      call qv_check(qv_host_free(c_loc(a)), 'qv_host_free')
+    else
+      deallocate(a)
+    end if
+    nullify(a)
   end subroutine
 
   !> room for pinsert: one new point per slot at most, so 2 pcount slots
   subroutine ensure_capacity()
     integer :: cap, old
-    real(c_double), allocatable :: t3(:,:), t1(:)
-    integer(c_int32_t), allocatable :: ti(:)
     old = size(infront)
     if (old >= 2 * pcount) return
     cap = 2 * pcount + 64
     call grow3(x); call grow3(u); call grow3(u1); call grow3(u2); call grow3(u3)
     call grow3(gi); call grow3(gb); call grow3(gii); call grow3(gbb)
     call growi(infront); call growi(behind); call growi(closest)
-    allocate(t1(cap)); t1 = 0._c_double; t1(1:old) = closestd(1:old)
-    call move_alloc(t1, closestd)
+    call grow1(closestd)
   contains
     subroutine grow3(a)
-      real(c_double), allocatable, intent(inout) :: a(:,:)
-      allocate(t3(3, cap)); t3 = 0._c_double; t3(:, 1:old) = a(:, 1:old)
-      call move_alloc(t3, a)
+      real(c_double), pointer, contiguous, intent(inout) :: a(:,:)
+      real(c_double), pointer, contiguous :: t(:,:)
+      call new3(t, cap)
+      t(:, 1:old) = a(:, 1:old)
+      call free3(a)
+      a => t
+    end subroutine
+    subroutine grow1(a)
+      real(c_double), pointer, contiguous, intent(inout) :: a(:)
+      real(c_double), pointer, contiguous :: t(:)
+      call new1(t, cap)
+      t(1:old) = a(1:old)
+      call free1(a)
+      a => t
     end subroutine
     subroutine growi(a)
-      integer(c_int32_t), allocatable, intent(inout) :: a(:)
-      allocate(ti(cap)); ti = 0; ti(1:old) = a(1:old)
-      call move_alloc(ti, a)
+      integer(c_int32_t), pointer, contiguous, intent(inout) :: a(:)
+      integer(c_int32_t), pointer, contiguous :: t(:)
+      call newi(t, cap)
+      t(1:old) = a(1:old)
+      call freei(a)
+      a => t
     end subroutine
   end subroutine
 

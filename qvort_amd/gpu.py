@@ -37,6 +37,7 @@ EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_h
            "qv_batch_destroy", "qv_batch_upload", "qv_batch_velocity", "qv_batch_step",
            "qv_batch_download", "qv_batch_tune", "qv_batch_set_variant", "qv_batch_info",
            "qv_download_ghosts", "qv_line_pinsert", "qv_line_precon", "qv_line_premove",
+           "qv_host_alloc", "qv_host_free",
            "qv_status_string"]
 
 
