@@ -4,7 +4,11 @@
 //
 // Wavefront-cooperative DFS.  A wavefront owns 64 targets that are adjacent
 // on a Hilbert curve through the box, so their walks overlap (4% fewer wave
-// steps than 64 targets adjacent in the tree's Morton order).  The walk state is
+// steps than 64 targets adjacent in the tree's Morton order).  Each group of
+// targets is two work items, launched as one grid: the central shell's walk
+// (about two thirds of the steps) in the first half of the blocks, the image
+// shells' in the second, so that shorter items fill the GPU as the launch
+// drains; k_walk_join composes the two sums.  The walk state is
 // (node, end of its sibling list, lane mask), uniform across the wave: the
 // current entry lives in scalar registers and an LDS stack keeps the sibling
 // lists left behind when the walk descends.  Every node record (64 bytes)
@@ -56,6 +60,7 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 
 struct WalkArgs {
   int nact;
+  int groups;  // blocks of one part: the central shell's walks, then the images'
   int64_t nnodes;
   double qc, theta;
   int extra;
@@ -300,15 +305,20 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
 template <int kMode>
 __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
+  // blocks [0, groups) walk the central shell of their targets, blocks
+  // [groups, 2 groups) the image shells of the same targets: two shorter work
+  // items per wavefront's targets, the long ones first, so that the last
+  // blocks leave less of the GPU idle
+  const int G = A.groups;
+  const bool images = (int)blockIdx.x >= G;
+  const int b = (int)blockIdx.x - (images ? G : 0);
   // XCD-aware order: consecutive blocks of the target list go to one XCD
-  const int G = gridDim.x;
-  const int per = (G + 7) / 8;
-  const int b = blockIdx.x;
+  const int per = G / 8;
   const int lb = (b % 8) * per + (b / 8);
   const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const int w = lb * kBlock + threadIdx.x;
-  const bool valid = lb < G && w < A.nact;
+  const bool valid = w < A.nact;
   int t = 0, bi = -1, fi = -1;
   double xi[3] = {0., 0., 0.};
   if (valid) {
@@ -322,22 +332,23 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   if (active == 0) return;
   WalkCounts wc;
   StackEntry *st = stack[wid];
-  // central shell, then the image shells into a second sum
+  // the central shell's sum, or the image shells' second sum
   // (timestep.f90:276-288)
-  double uc[3] = {0., 0., 0.};
+  double r[3] = {0., 0., 0.};
   {
-    const double z[3] = {0., 0., 0.};
     double u[3] = {0., 0., 0.};
     AccF64 acc;
-    walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, st, u, acc, wc);
-    if (kMode == QV_FP64) { uc[0] = u[0]; uc[1] = u[1]; uc[2] = u[2]; }
-    else acc.get(uc[0], uc[1], uc[2]);
-  }
-  double u[3] = {0., 0., 0.};
-  AccF64 acc;
-  for (int sh = 1; sh < S.n; ++sh) {
-    const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, st, u, acc, wc);
+    if (!images) {
+      const double z[3] = {0., 0., 0.};
+      walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, st, u, acc, wc);
+    } else {
+      for (int sh = 1; sh < S.n; ++sh) {
+        const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
+        walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, st, u, acc, wc);
+      }
+    }
+    if (kMode == QV_FP64) { r[0] = u[0]; r[1] = u[1]; r[2] = u[2]; }
+    else acc.get(r[0], r[1], r[2]);
   }
   // the output pointers are read from the kernel-argument segment here, after
   // the walk, so that they do not occupy scalar registers during it
@@ -352,26 +363,35 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   }
   if (wc.singular) atomicOr(O.flags, kFlagSingular);
   if (!valid) return;
-  double us[3] = {0., 0., 0.};
-  if (kMode == QV_FP64) {
-    us[0] = u[0]; us[1] = u[1]; us[2] = u[2];
+  // raw sums for k_walk_join: the central one in u_bs, the images' in u
+  if (!images) {
+    O.bx[t] = r[0]; O.by[t] = r[1]; O.bz[t] = r[2];
   } else {
-    acc.get(us[0], us[1], us[2]);
+    O.ux[t] = r[0]; O.uy[t] = r[1]; O.uz[t] = r[2];
   }
+}
+
+// calc_velocity's composition (timestep.f90:273-288) from the two sums
+__global__ void __launch_bounds__(kBlock) k_walk_join(WalkArgs A, int nshells) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.nact) return;
+  const int t = A.slots[s];
+  double uc[3] = {A.bx[t], A.by[t], A.bz[t]};
+  double us[3] = {0., 0., 0.};
+  if (nshells > 1) { us[0] = A.ux[t]; us[1] = A.uy[t]; us[2] = A.uz[t]; }
   // u_bs * quant_circ / 4 / pi (tree.f90:390, 411), once per sum
   for (int k = 0; k < 3; ++k) {
-    uc[k] = ((uc[k] * O.qc) / 4.) / kPi;
-    us[k] = ((us[k] * O.qc) / 4.) / kPi;
+    uc[k] = ((uc[k] * A.qc) / 4.) / kPi;
+    us[k] = ((us[k] * A.qc) / 4.) / kPi;
   }
-  // calc_velocity composition (timestep.f90:273-288)
-  double u0 = O.ulx[t] + uc[0], u1 = O.uly[t] + uc[1], u2 = O.ulz[t] + uc[2];
+  double u0 = A.ulx[t] + uc[0], u1 = A.uly[t] + uc[1], u2 = A.ulz[t] + uc[2];
   double b0 = uc[0], b1 = uc[1], b2 = uc[2];
-  if (S.n > 1) {
+  if (nshells > 1) {
     u0 = u0 + us[0]; u1 = u1 + us[1]; u2 = u2 + us[2];
     b0 = b0 + us[0]; b1 = b1 + us[1]; b2 = b2 + us[2];
   }
-  O.ux[t] = u0; O.uy[t] = u1; O.uz[t] = u2;
-  O.bx[t] = b0; O.by[t] = b1; O.bz[t] = b2;
+  A.ux[t] = u0; A.uy[t] = u1; A.uz[t] = u2;
+  A.bx[t] = b0; A.by[t] = b1; A.bz[t] = b2;
 }
 
 }  // namespace
@@ -399,10 +419,16 @@ hipError_t launch_walk(Ctx &c) {
   Shells S = make_shells(c.p);
   int g = (c.nact + kBlock - 1) / kBlock;
   g = ((g + 7) / 8) * 8;
+  A.groups = g;
+  const dim3 grid(S.n > 1 ? 2 * g : g);
   if (c.p.precision == QV_FP32C)
-    hipLaunchKernelGGL((k_walk<QV_FP32C>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
+    hipLaunchKernelGGL((k_walk<QV_FP32C>), grid, dim3(kBlock), 0, c.stream, A, S);
   else
-    hipLaunchKernelGGL((k_walk<QV_FP64>), dim3(g), dim3(kBlock), 0, c.stream, A, S);
+    hipLaunchKernelGGL((k_walk<QV_FP64>), grid, dim3(kBlock), 0, c.stream, A, S);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_walk_join, dim3((c.nact + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream,
+                     A, S.n);
   return hipGetLastError();
 }
 
