@@ -74,6 +74,7 @@ int reserve(Ctx &c, int32_t n) {
   TRY(dalloc(c.slot, cap));
   TRY(dalloc(c.slot_s, cap));
   TRY(dalloc(c.walk_order, cap));
+  TRY(dalloc(c.walk_part, 3 * cap));
   TRY(dalloc(c.nnode_of, cap + 1));
   TRY(dalloc(c.node_off, cap + 1));
   c.cap = (int32_t)cap;
@@ -311,7 +312,8 @@ int qv_destroy(qv_ctx *ctx) {
                 &c->u1, &c->u2, &c->u3, &c->xs, &c->gs, &c->aw, &c->ar};
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
-                c->slot_s, c->walk_order, c->nnode_of, c->node_off, c->sort_tmp, c->counters, c->evals,
+                c->slot_s, c->walk_order, c->walk_part, c->nnode_of, c->node_off, c->sort_tmp,
+                c->counters, c->evals,
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,

@@ -111,6 +111,8 @@ struct Ctx {
   // the walk's target order: sorted positions in Hilbert-curve order, so
   // that the 64 targets of a wavefront are spatially compact
   int32_t *walk_order = nullptr;
+  // the walk's raw sums of the second half of the image shells, 3 per slot
+  double *walk_part = nullptr;
   int32_t *nnode_of = nullptr, *node_off = nullptr;  // per sorted position
   Vec3 xs, gs;  // positions and ghosts in sorted order
   void *sort_tmp = nullptr;

@@ -5,10 +5,12 @@
 // Wavefront-cooperative DFS.  A wavefront owns 64 targets that are adjacent
 // on a Hilbert curve through the box, so their walks overlap (4% fewer wave
 // steps than 64 targets adjacent in the tree's Morton order).  Each group of
-// targets is two work items, launched as one grid: the central shell's walk
-// (about two thirds of the steps) in the first half of the blocks, the image
-// shells' in the second, so that shorter items fill the GPU as the launch
-// drains; k_walk_join composes the two sums.  The walk state is
+// targets is three work items, launched as one grid: the central shell's walk
+// (about two thirds of the steps) in the first third of the blocks, the first
+// half of the image shells in the second, the other half in the last, so that
+// shorter items fill the GPU as the launch drains; k_walk_join composes the
+// sums (the two image halves added before the image sum joins the central
+// one, as calc_velocity adds its image sum, timestep.f90:280-288).  The walk state is
 // (node, end of its sibling list, lane mask), uniform across the wave: the
 // current entry lives in scalar registers and an LDS stack keeps the sibling
 // lists left behind when the walk descends.  Every node record (64 bytes)
@@ -74,6 +76,7 @@ struct WalkArgs {
   const double2 *geo;
   const double *ulx, *uly, *ulz;    // slot order
   double *ux, *uy, *uz, *bx, *by, *bz;
+  double *p2;  // the second half of the image shells' raw sums, 3 per slot
   unsigned long long *evals;  // [0] leaf + cell interactions, [1] lane visits, [2] leaves,
                               // [3] wave steps (nodes fetched by a wavefront)
   int32_t *flags;
@@ -306,12 +309,13 @@ template <int kMode>
 __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
   // blocks [0, groups) walk the central shell of their targets, blocks
-  // [groups, 2 groups) the image shells of the same targets: two shorter work
-  // items per wavefront's targets, the long ones first, so that the last
-  // blocks leave less of the GPU idle
+  // [groups, 2 groups) and [2 groups, 3 groups) the two halves of the image
+  // shells of the same targets: shorter work items per wavefront's targets,
+  // the long ones first, so that the last blocks leave less of the GPU idle
   const int G = A.groups;
-  const bool images = (int)blockIdx.x >= G;
-  const int b = (int)blockIdx.x - (images ? G : 0);
+  const int part = (int)blockIdx.x / G;
+  const bool images = part > 0;
+  const int b = (int)blockIdx.x - part * G;
   // XCD-aware order: consecutive blocks of the target list go to one XCD
   const int per = G / 8;
   const int lb = (b % 8) * per + (b / 8);
@@ -342,7 +346,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
       const double z[3] = {0., 0., 0.};
       walk_shell<kMode, false>(A, z, xi, t, bi, fi, active, st, u, acc, wc);
     } else {
-      for (int sh = 1; sh < S.n; ++sh) {
+      const int half = 1 + (S.n - 1) / 2;
+      const int sh0 = part == 1 ? 1 : half, sh1 = part == 1 ? half : S.n;
+      for (int sh = sh0; sh < sh1; ++sh) {
         const double sv[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
         walk_shell<kMode, true>(A, sv, xi, t, bi, fi, active, st, u, acc, wc);
       }
@@ -363,11 +369,14 @@ __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   }
   if (wc.singular) atomicOr(O.flags, kFlagSingular);
   if (!valid) return;
-  // raw sums for k_walk_join: the central one in u_bs, the images' in u
-  if (!images) {
+  // raw sums for k_walk_join: the central one in u_bs, the images' halves in
+  // u and in the part buffer
+  if (part == 0) {
     O.bx[t] = r[0]; O.by[t] = r[1]; O.bz[t] = r[2];
-  } else {
+  } else if (part == 1) {
     O.ux[t] = r[0]; O.uy[t] = r[1]; O.uz[t] = r[2];
+  } else {
+    O.p2[3 * (int64_t)t] = r[0]; O.p2[3 * (int64_t)t + 1] = r[1]; O.p2[3 * (int64_t)t + 2] = r[2];
   }
 }
 
@@ -378,7 +387,11 @@ __global__ void __launch_bounds__(kBlock) k_walk_join(WalkArgs A, int nshells) {
   const int t = A.slots[s];
   double uc[3] = {A.bx[t], A.by[t], A.bz[t]};
   double us[3] = {0., 0., 0.};
-  if (nshells > 1) { us[0] = A.ux[t]; us[1] = A.uy[t]; us[2] = A.uz[t]; }
+  if (nshells > 1) {
+    us[0] = A.ux[t]; us[1] = A.uy[t]; us[2] = A.uz[t];
+    if (nshells > 2)
+      for (int k = 0; k < 3; ++k) us[k] = us[k] + A.p2[3 * (int64_t)t + k];
+  }
   // u_bs * quant_circ / 4 / pi (tree.f90:390, 411), once per sum
   for (int k = 0; k < 3; ++k) {
     uc[k] = ((uc[k] * A.qc) / 4.) / kPi;
@@ -420,7 +433,8 @@ hipError_t launch_walk(Ctx &c) {
   int g = (c.nact + kBlock - 1) / kBlock;
   g = ((g + 7) / 8) * 8;
   A.groups = g;
-  const dim3 grid(S.n > 1 ? 2 * g : g);
+  A.p2 = c.walk_part;
+  const dim3 grid(S.n > 2 ? 3 * g : (S.n > 1 ? 2 * g : g));
   if (c.p.precision == QV_FP32C)
     hipLaunchKernelGGL((k_walk<QV_FP32C>), grid, dim3(kBlock), 0, c.stream, A, S);
   else
