@@ -151,6 +151,37 @@ def roofline(res, precision: str, n: int, traffic):
 PAIR_FLOPS = 52  # one FP32 segment term, counting FMA as 2: see run_ensemble
 
 
+DIRECT_PAIR_FLOPS = 45  # one desingularised segment term (a6 form), +,-,*,/,sqrt as one
+
+
+def run_direct(n_points: int = 262144, steps: int = 3) -> dict:
+    """The tiled direct sum (velocity='BS', biot_savart, timestep.f90:462-482)
+    of one system of random loops in an open box, FP64 and FP32C, against
+    the vector FLOP peaks: 45 algorithmic flops per source-target pair."""
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    fil = G.random_loops(0.1, 5e-4, target_points=n_points, seed=7, spacing=0.8, wrap=False)
+    n = int(np.count_nonzero(fil.infront))
+    pairs = n * (n - 2)  # every active source but the target and its behind
+    out = {"workload": "random loops, open box L=0.1 cm, delta=5e-4 cm, velocity BS",
+           "points": n, "pairs": pairs, "pair_flops": DIRECT_PAIR_FLOPS}
+    for prec in ("fp64", "fp32c"):
+        p = gpu.params(velocity="BS", tree_theta=0.0, boundary="open", box_size=0.1, delta=5e-4,
+                       corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6, precision=prec)
+        with gpu.Context(p) as c:
+            c.upload_filaments(fil)
+            c.velocity_only()
+            ms = []
+            for _ in range(steps):
+                c.velocity_only()
+                ms.append(c.timings()["walk_ms"])
+        t = float(np.mean(ms))
+        tf = DIRECT_PAIR_FLOPS * pairs / (t * 1e-3) / 1e12
+        out[prec] = {"kernel_ms": round(t, 3), "achieved_tflops": round(tf, 2),
+                     "peak_tflops": FLOP_PEAK_TFS[prec], "frac": round(tf / FLOP_PEAK_TFS[prec], 4)}
+    return out
+
+
 def run_ensemble(steps: int, seed: int = 20261202):
     """Ensemble of 256 rings advanced together (qv_batch_*).  dt is half the
     stability bound of timestep_check (initial.f90:606-622) for the finest
@@ -268,6 +299,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-next-rows", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32c"], default=None,
                     help="profile one precision (no JSON line)")
@@ -343,6 +375,9 @@ def main():
         "roofline": roofline(r32, "fp32c", n, traffic),
         "roofline_fp64": roofline(r64, "fp64", n, None),
     }
+    if not args.no_direct:
+        out["direct_sum"] = run_direct()
+        print(f"direct sum: {out['direct_sum']}", file=sys.stderr, flush=True)
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.ensemble_steps)
         print(f"ensemble: {out['ensemble']['ms_per_step']:.3f} ms/step", file=sys.stderr,

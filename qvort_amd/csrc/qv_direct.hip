@@ -6,7 +6,8 @@
 // called from :227-259), and keeps their operation order, so its sums are
 // bit-identical to the reference.  The FP32C kernel evaluates each pair with
 // the equivalent segment form of tree_walk's leaf (tree.f90:380-392) in
-// FP32, relative vectors formed in FP64, and sums per target in FP64.
+// FP32 with fused multiply-adds, relative vectors formed in FP64, and sums
+// per target in FP64.
 #include "qv_internal.h"
 
 #pragma clang fp contract(off)
@@ -62,6 +63,15 @@ __device__ inline void bs_pair(const double xi[3], const double xj[3], const dou
 struct SrcTile {
   double x[kTile], y[kTile], z[kTile];
   double gx[kTile], gy[kTile], gz[kTile];
+  int32_t ok[kTile];
+};
+
+// the FP32C kernel's tile: positions in FP64 (the relative vector is formed
+// in FP64), the segment g - x already rounded to FP32 (it does not depend
+// on the target)
+struct SrcTile32 {
+  double x[kTile], y[kTile], z[kTile];
+  float ex[kTile], ey[kTile], ez[kTile];
   int32_t ok[kTile];
 };
 
@@ -129,34 +139,47 @@ __global__ void __launch_bounds__(kBlock) k_direct_fp64(DirectArgs A, Shells S) 
   A.bx[i] = b0; A.by[i] = b1; A.bz[i] = b2;
 }
 
-// FP32 segment term, R1 = x_j+s-x_i and the segment e = g_j - x_j given
-// (both formed in FP64 and rounded once).  Straight-segment Biot-Savart in
-// the form of tree.f90:380-392 with R1 x R2 = R1 x e and, for targets that
-// project inside the segment, ab + R1.R2 = |R1 x R2|^2 / (ab - R1.R2), which
-// keeps FP32 free of cancellation.  Returns false for the degenerate case
-// |R1 x e| == 0 that the FP64 path skips through D == 0.
-__device__ inline bool seg32(float r0, float r1, float r2, float e0, float e1, float e2,
-                             float &w0, float &w1, float &w2) {
-  const float c0 = r1 * e2 - r2 * e1, c1 = r2 * e0 - r0 * e2, c2 = r0 * e1 - r1 * e0;
-  const float cc = c0 * c0 + c1 * c1 + c2 * c2;
-  if (cc == 0.f) return false;
+__device__ inline void load_tile32(SrcTile32 &t, const DirectArgs &A, int base) {
+  const int j = base + threadIdx.x;
+  if (threadIdx.x < kTile) {
+    const bool in = j < A.n;
+    const double x = in ? A.X[j] : 0., y = in ? A.Y[j] : 0., z = in ? A.Z[j] : 0.;
+    t.x[threadIdx.x] = x;
+    t.y[threadIdx.x] = y;
+    t.z[threadIdx.x] = z;
+    t.ex[threadIdx.x] = in ? (float)(A.GX[j] - x) : 0.f;
+    t.ey[threadIdx.x] = in ? (float)(A.GY[j] - y) : 0.f;
+    t.ez[threadIdx.x] = in ? (float)(A.GZ[j] - z) : 0.f;
+    t.ok[threadIdx.x] = in ? (A.inf[j] != 0) : 0;
+  }
+}
+
+__device__ inline float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// seg32 with fused multiply-adds (the FP32 path has no reference rounding to
+// keep), as a weight and a direction: the term is W (c0, c1, c2) with W = 0
+// where `on` is false or |R1 x e| == 0 (the case the FP64 path skips)
+__device__ inline void seg32w(bool on, float r0, float r1, float r2, float e0, float e1,
+                              float e2, float &W, float &c0, float &c1, float &c2) {
+  c0 = fmaf_(r1, e2, -r2 * e1);
+  c1 = fmaf_(r2, e0, -r0 * e2);
+  c2 = fmaf_(r0, e1, -r1 * e0);
+  const float cc = fmaf_(c2, c2, fmaf_(c1, c1, c0 * c0));
   const float s0 = r0 + e0, s1 = r1 + e1, s2 = r2 + e2;
-  const float a2 = r0 * r0 + r1 * r1 + r2 * r2;
-  const float b2 = s0 * s0 + s1 * s1 + s2 * s2;
+  const float a2 = fmaf_(r2, r2, fmaf_(r1, r1, r0 * r0));
+  const float b2 = fmaf_(s2, s2, fmaf_(s1, s1, s0 * s0));
   const float ia = __builtin_amdgcn_rsqf(a2), ib = __builtin_amdgcn_rsqf(b2);
   const float a = a2 * ia, b = b2 * ib;
   const float ab = a * b;
-  const float dt = r0 * s0 + r1 * s1 + r2 * s2;
-  const float g = (a + b) * ia * ib;
+  const float dt = fmaf_(r2, s2, fmaf_(r1, s1, r0 * s0));
+  const float g = (a + b) * (ia * ib);
   const float f = dt >= 0.f ? g * __builtin_amdgcn_rcpf(ab + dt)
                             : g * (ab - dt) * __builtin_amdgcn_rcpf(cc);
-  w0 = f * c0; w1 = f * c1; w2 = f * c2;
-  return true;
+  W = (on && cc != 0.f) ? f : 0.f;
 }
 
-template <class Acc>
 __global__ void __launch_bounds__(kBlock) k_direct_fp32c(DirectArgs A, Shells S) {
-  __shared__ SrcTile t;
+  __shared__ SrcTile32 t;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < A.n && A.inf[i] != 0;
   double xi[3] = {0., 0., 0.};
@@ -165,30 +188,37 @@ __global__ void __launch_bounds__(kBlock) k_direct_fp32c(DirectArgs A, Shells S)
     xi[0] = A.X[i]; xi[1] = A.Y[i]; xi[2] = A.Z[i];
     bi = A.beh[i] - 1;
   }
+  // per-target sums in FP64 (on CDNA4 an FP64 fused multiply-add issues at
+  // the FP32 rate: the cheapest compensated sum of FP32 terms)
   double uc[3] = {0., 0., 0.}, us[3] = {0., 0., 0.};
-  Acc ac, as;
   for (int sh = 0; sh < S.n; ++sh) {
     const double xs0 = xi[0] - S.s[sh][0], xs1 = xi[1] - S.s[sh][1], xs2 = xi[2] - S.s[sh][2];
+    double v0 = 0., v1 = 0., v2 = 0.;
     for (int base = 0; base < A.n; base += kTile) {
       __syncthreads();
-      load_tile(t, A, base);
+      load_tile32(t, A, base);
       __syncthreads();
       if (!act) continue;
       const int m = min(kTile, A.n - base);
       for (int jj = 0; jj < m; ++jj) {
         const int j = base + jj;
-        if (!t.ok[jj] || j == bi || (sh == 0 && j == i)) continue;
+        const bool on = t.ok[jj] && j != bi && !(sh == 0 && j == i);
         const float r0 = (float)(t.x[jj] - xs0), r1 = (float)(t.y[jj] - xs1),
                     r2 = (float)(t.z[jj] - xs2);
-        const float e0 = (float)(t.gx[jj] - t.x[jj]), e1 = (float)(t.gy[jj] - t.y[jj]),
-                    e2 = (float)(t.gz[jj] - t.z[jj]);
-        float w0, w1, w2;
-        if (seg32(r0, r1, r2, e0, e1, e2, w0, w1, w2)) (sh == 0 ? ac : as).add(w0, w1, w2);
+        float W, c0, c1, c2;
+        seg32w(on, r0, r1, r2, t.ex[jj], t.ey[jj], t.ez[jj], W, c0, c1, c2);
+        const double w = (double)W;
+        v0 = fma(w, (double)c0, v0);
+        v1 = fma(w, (double)c1, v1);
+        v2 = fma(w, (double)c2, v2);
       }
     }
+    if (sh == 0) {
+      uc[0] = v0; uc[1] = v1; uc[2] = v2;
+    } else {
+      us[0] = us[0] + v0; us[1] = us[1] + v1; us[2] = us[2] + v2;
+    }
   }
-  ac.get(uc[0], uc[1], uc[2]);
-  as.get(us[0], us[1], us[2]);
   if (i >= A.n) return;
   if (!act) {
     A.ux[i] = A.uy[i] = A.uz[i] = 0.;
@@ -222,7 +252,7 @@ hipError_t launch_direct(Ctx &c) {
   Shells S = make_shells(c.p);
   const int g = (c.pcount + kBlock - 1) / kBlock;
   if (c.p.precision == QV_FP32C)
-    hipLaunchKernelGGL(k_direct_fp32c<AccKahan>, dim3(g), dim3(kBlock), 0, c.stream, A, S);
+    hipLaunchKernelGGL(k_direct_fp32c, dim3(g), dim3(kBlock), 0, c.stream, A, S);
   else
     hipLaunchKernelGGL(k_direct_fp64, dim3(g), dim3(kBlock), 0, c.stream, A, S);
   return hipGetLastError();

@@ -140,34 +140,16 @@ enum : int32_t { kFlagSingular = 1, kFlagDepth = 2 };
 
 Shells make_shells(const qv_params &p);
 
-// per-target accumulators of FP32 terms.  AccF64 folds every term into an
-// FP64 sum (on CDNA4 an FP64 add costs the same issue slot as an FP32 add,
-// so this is the cheapest compensated sum); AccKahan keeps the FP32 sum with
-// a Kahan-Babuska (Neumaier) compensation term.
+// per-target accumulator of FP32 terms: every term folded into an FP64 sum
+// (on CDNA4 an FP64 add costs the same issue slot as an FP32 add, so this is
+// the cheapest compensated sum; Kahan or two-sum in FP32 costs three times
+// the instructions per term).
 struct AccF64 {
   double s0 = 0., s1 = 0., s2 = 0.;
   __device__ inline void add(float a, float b, float c) {
     s0 += (double)a; s1 += (double)b; s2 += (double)c;
   }
   __device__ inline void get(double &a, double &b, double &c) const { a = s0; b = s1; c = s2; }
-};
-
-struct AccKahan {
-  float s[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
-  __device__ inline void add1(int k, float v) {
-    const float t = s[k] + v;
-    // two-sum error term without a branch (Knuth): exact when no overflow
-    const float bp = t - s[k];
-    const float ap = t - bp;
-    e[k] += (s[k] - ap) + (v - bp);
-    s[k] = t;
-  }
-  __device__ inline void add(float a, float b, float c) { add1(0, a); add1(1, b); add1(2, c); }
-  __device__ inline void get(double &a, double &b, double &c) const {
-    a = (double)s[0] + (double)e[0];
-    b = (double)s[1] + (double)e[1];
-    c = (double)s[2] + (double)e[2];
-  }
 };
 
 // launchers (each returns hipError_t of the launch)
