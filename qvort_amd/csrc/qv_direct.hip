@@ -173,8 +173,9 @@ __device__ inline void seg32w(bool on, float r0, float r1, float r2, float e0, f
   const float ab = a * b;
   const float dt = fmaf_(r2, s2, fmaf_(r1, s1, r0 * s0));
   const float g = (a + b) * (ia * ib);
-  const float f = dt >= 0.f ? g * __builtin_amdgcn_rcpf(ab + dt)
-                            : g * (ab - dt) * __builtin_amdgcn_rcpf(cc);
+  // one reciprocal, of the selected denominator
+  const bool in = dt >= 0.f;
+  const float f = (in ? g : g * (ab - dt)) * __builtin_amdgcn_rcpf(in ? ab + dt : cc);
   W = (on && cc != 0.f) ? f : 0.f;
 }
 

@@ -145,13 +145,10 @@ __device__ inline void leaf32(bool on, double r0d, double r1d, double r2d, doubl
   const float ab = a * b;
   const float dt = fmaf_(r2, s2, fmaf_(r1, s1, r0 * s0));
   const float g = (a + b) * (ia * ib);
-  float f;
-  if (dt >= 0.f) {
-    f = g * __builtin_amdgcn_rcpf(ab + dt);
-  } else {
-    const float cc = fmaf_(c2, c2, fmaf_(c1, c1, c0 * c0));
-    f = g * (ab - dt) * __builtin_amdgcn_rcpf(cc);
-  }
+  // one reciprocal, of the selected denominator
+  const bool in = dt >= 0.f;
+  const float cc = fmaf_(c2, c2, fmaf_(c1, c1, c0 * c0));
+  const float f = (in ? g : g * (ab - dt)) * __builtin_amdgcn_rcpf(in ? ab + dt : cc);
   // lanes that are off may hold inf/nan in f (e.g. their own point); R1 x e
   // is finite, so selecting f alone zeroes their contribution
   W = on ? f : 0.f;
