@@ -107,11 +107,12 @@ def test_fp32c_velocity_vs_golden(name):
 
 
 @pytest.mark.parametrize("theta,boundary", [(0.3, "periodic"), (0.4, "periodic"),
-                                            (0.6, "periodic"), (0.4, "open")])
+                                            (0.6, "periodic"), (0.4, "open"), (0.4, "openx"),
+                                            (0.5, "openxy")])
 def test_fp64_tree_vs_oracle_20k(theta, boundary):
     from oracle import port
     from qvort_amd import geometry as G
-    fil = G.random_loops(0.1, 5e-4, target_points=20000, seed=31, wrap=boundary == "periodic")
+    fil = G.random_loops(0.1, 5e-4, target_points=20000, seed=31, wrap=boundary != "open")
     kw = dict(velocity="Tree", tree_theta=theta, boundary=boundary, box_size=0.1,
               corea=1.3e-8)
     _say("oracle", fil.pcount, theta, boundary)
