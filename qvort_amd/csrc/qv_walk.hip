@@ -299,9 +299,10 @@ __device__ inline void walk_shell(const WalkArgs &A, const double sv[3], const d
   }
 }
 
-// 8 wavefronts per SIMD: the register budget that leaves (53 VGPRs, 78
-// SGPRs, no spills in the FP32C node loop) hides more of the node loads'
-// latency than 7 waves with spare registers
+// at least 8 wavefronts per SIMD: the register budget that leaves hides more
+// of the node loads' latency than 7 waves with spare registers (the FP32C
+// walk now takes 47 VGPRs and 78 SGPRs, no spills in its node loop, which
+// leaves room for 10)
 template <int kMode>
 __global__ void __launch_bounds__(kBlock, 8) k_walk(WalkArgs A, Shells S) {
   __shared__ StackEntry stack[kWavesPerBlock][kStackDepth];
