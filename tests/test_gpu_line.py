@@ -97,3 +97,32 @@ def test_fortran_driver_with_line_operations(name, tmp_path):
     ref = g["final__u1"][keep]
     assert np.max(np.linalg.norm(f["u"][keep] - ref, axis=1) / np.linalg.norm(ref, axis=1)) <= 1e-12
     assert not np.any(f["u"][~act])
+
+
+def test_pinned_host_memory_round_trip():
+    """qv_host_alloc gives page-locked memory that qv_upload / qv_download use
+    as any host array; qv_host_free releases it (NULL is ignored)."""
+    import ctypes as C
+    from qvort_amd import geometry as G
+    from qvort_amd import gpu
+    L = gpu.lib()
+    L.qv_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    L.qv_host_alloc.restype = C.c_int
+    L.qv_host_free.argtypes = [C.c_void_p]
+    L.qv_host_free.restype = C.c_int
+    fil = G.random_loops(0.1, 5e-4, n_loops=3, seed=61)
+    n = fil.pcount
+    ptr = C.c_void_p()
+    assert L.qv_host_alloc(24 * n, C.byref(ptr)) == 0 and ptr.value
+    try:
+        buf = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(n, 3))
+        buf[:] = fil.x
+        kw = dict(velocity="BS", tree_theta=0.0, boundary="periodic", box_size=0.1, corea=1.3e-8)
+        with gpu.Context(gpu.params(**kw)) as c:
+            c.upload(buf, fil.infront, fil.behind)
+            buf[:] = 0.
+            x = c.download(out=[buf] + [np.zeros((n, 3)) for _ in range(4)])[0]
+        assert np.array_equal(buf, fil.x) and np.array_equal(x, fil.x)
+    finally:
+        assert L.qv_host_free(ptr) == 0
+    assert L.qv_host_free(None) == 0
