@@ -60,6 +60,8 @@ hipError_t vzero(Vec3 &v, size_t n, hipStream_t s) {
 // pcount creep up from one upload to the next
 int reserve(Ctx &c, int32_t n) {
   if (n <= c.cap) return QV_OK;
+  // the buffers below are freed: the walk order's sort may still use some
+  if (c.aux_stream) TRY(hipStreamSynchronize(c.aux_stream));
   const size_t cap = (size_t)n + (size_t)n / 4 + 1024;
   if (cap > (size_t)INT32_MAX) return QV_ERR_ARG;
   Vec3 *vs[] = {&c.x, &c.gi, &c.gb, &c.gii, &c.gbb, &c.u, &c.uli, &c.ubs,
@@ -291,6 +293,12 @@ int qv_create(const qv_params *params, qv_ctx **out) {
       return QV_ERR_HIP;
     }
   }
+  if (hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->aux_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->aux_ev[1], hipEventDisableTiming) != hipSuccess) {
+    qv_destroy(reinterpret_cast<qv_ctx *>(c));
+    return QV_ERR_HIP;
+  }
   if (hipMalloc(&c->counters, 8 * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->evals, 4 * sizeof(int64_t)) != hipSuccess ||
       hipHostMalloc(&c->h_counters, 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
@@ -308,6 +316,7 @@ int qv_destroy(qv_ctx *ctx) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->own_stream && c->own_stream != c->stream) (void)hipStreamSynchronize(c->own_stream);
+  if (c->aux_stream) (void)hipStreamSynchronize(c->aux_stream);
   Vec3 *vs[] = {&c->x, &c->gi, &c->gb, &c->gii, &c->gbb, &c->u, &c->uli, &c->ubs,
                 &c->u1, &c->u2, &c->u3, &c->xs, &c->gs, &c->aw, &c->ar};
   for (Vec3 *v : vs) vfree(*v);
@@ -317,13 +326,17 @@ int qv_destroy(qv_ctx *ctx) {
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
-                c->tree.linf, c->close_i, c->close_d, c->close_order, c->probe_buf, c->probe_part};
+                c->tree.linf, c->close_i, c->close_d, c->close_order, c->probe_buf, c->probe_part,
+                c->aux_tmp};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->h_evals) (void)hipHostFree(c->h_evals);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (c->aux_stream) (void)hipStreamDestroy(c->aux_stream);
+  for (auto &e : c->aux_ev)
+    if (e) (void)hipEventDestroy(e);
   delete c;
   return QV_OK;
 }
@@ -519,7 +532,9 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
   TRY(hipMemcpyAsync(c->close_d, closestd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (tree) {
     // the search's slot order; the slot buffers hold cap >= pcount entries
-    const int rc = hilbert_order(*c, c->pcount, c->x, c->infront, c->close_order);
+    int rc = join_aux(*c);
+    if (!rc) rc = hilbert_order(*c, c->pcount, c->x, c->infront, c->close_order, c->stream,
+                                c->sort_tmp, c->sort_tmp_bytes);
     if (rc) return rc;
   }
   TRY(launch_closest(*c, tree));

@@ -133,6 +133,14 @@ struct Ctx {
   int32_t adj_cap = 0;
   bool links_consistent = false;  // infront(behind(i)) == i for every point
   hipEvent_t ev[6] = {};
+  // a second stream (default priority) for the walk order's sort, which
+  // overlaps the rest of the tree build: aux_ev[0] = build data ready,
+  // aux_ev[1] = order sorted; aux_live once aux_ev[1] has been recorded
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t aux_ev[2] = {};
+  bool aux_live = false;
+  void *aux_tmp = nullptr;
+  size_t aux_tmp_bytes = 0;
 };
 
 // error flags accumulated in counters[1]
@@ -169,8 +177,13 @@ hipError_t launch_hilbert_keys(hipStream_t st, int n, double L, const double *x,
 // the probe walk's per-node thresholds (Tree::tplain) for the current tree
 hipError_t launch_plain_thresholds(Ctx &c);
 // slots 0..n-1 (positions pos) in Hilbert-curve order into order[0..n);
-// with inf, slots with infront = 0 go last.  Uses the build's key buffers.
-int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order);
+// with inf, slots with infront = 0 go last.  Uses the build's key buffers
+// and the sort scratch (tmp, tmp_bytes; grown as needed) on stream st.
+int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order, hipStream_t st,
+                  void *&tmp, size_t &tmp_bytes);
+// make c.stream wait for the walk order's sort on the second stream (it
+// reads and writes the build's key buffers)
+int join_aux(Ctx &c);
 int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
 hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);

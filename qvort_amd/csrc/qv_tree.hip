@@ -624,28 +624,34 @@ hipError_t launch_plain_thresholds(Ctx &c) {
   return e;
 }
 
-int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order) {
+int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order, hipStream_t st,
+                  void *&tmp, size_t &tmp_bytes) {
   if (n <= 0) return QV_OK;
   // the key and slot buffers of the build are free outside tree_build's
   // steps 1-4: keys in key0 (2 cap 32-bit words), indices in slot
   uint32_t *hk = reinterpret_cast<uint32_t *>(c.key0);
   uint32_t *hk_s = hk + c.cap;
-  hipStream_t st = c.stream;
   QV_TRY(launch_hilbert_keys(st, n, c.p.box_size, pos.x, pos.y, pos.z, 1, inf, hk, c.slot));
   const unsigned end_bit = inf ? 22 : 21;
   size_t need = 0;
   QV_TRY(rocprim::radix_sort_pairs<SortConfig>(nullptr, need, hk, hk_s, c.slot, order, n, 0,
                                                end_bit, st));
-  if (need > c.sort_tmp_bytes) {
-    (void)hipFree(c.sort_tmp);
-    c.sort_tmp = nullptr;
-    c.sort_tmp_bytes = 0;
-    QV_TRY(hipMalloc(&c.sort_tmp, need));
-    c.sort_tmp_bytes = need;
+  if (need > tmp_bytes) {
+    QV_TRY(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+    tmp = nullptr;
+    tmp_bytes = 0;
+    QV_TRY(hipMalloc(&tmp, need));
+    tmp_bytes = need;
   }
-  size_t have = c.sort_tmp_bytes;
-  QV_TRY(rocprim::radix_sort_pairs<SortConfig>(c.sort_tmp, have, hk, hk_s, c.slot, order, n, 0,
+  size_t have = tmp_bytes;
+  QV_TRY(rocprim::radix_sort_pairs<SortConfig>(tmp, have, hk, hk_s, c.slot, order, n, 0,
                                                end_bit, st));
+  return QV_OK;
+}
+
+int join_aux(Ctx &c) {
+  if (c.aux_live) QV_TRY(hipStreamWaitEvent(c.stream, c.aux_ev[1], 0));
   return QV_OK;
 }
 
@@ -653,6 +659,8 @@ int tree_build(Ctx &c) {
   const int n = c.pcount;
   const int nact = c.nact;
   hipStream_t st = c.stream;
+  // the previous build's sort on the second stream used key0 and slot
+  if (int rc = join_aux(c)) return rc;
   // 1. keys
   hipLaunchKernelGGL(k_keys, dim3(grid_of(n)), dim3(kBlock), 0, st, n, c.p.box_size, c.x.x,
                      c.x.y, c.x.z, c.infront, c.key0, c.key1, c.slot);
@@ -681,8 +689,18 @@ int tree_build(Ctx &c) {
                      c.key1, key1_s, c.x.x, c.x.y, c.x.z, c.gi.x, c.gi.y, c.gi.z, c.xs.x, c.xs.y,
                      c.xs.z, c.gs.x, c.gs.y, c.gs.z);
   QV_TRY(hipGetLastError());
-  // 4b. the walk's order of the sorted targets (Hilbert curve)
-  if (int rc = hilbert_order(c, nact, c.xs, nullptr, c.walk_order)) return rc;
+  // 4b. the walk's order of the sorted targets (Hilbert curve), on the second
+  // stream: it needs the sorted positions and key0/slot, which the steps
+  // below leave alone, and overlaps them; the stream joins before the walk
+  if (nact > 0) {
+    QV_TRY(hipEventRecord(c.aux_ev[0], st));
+    QV_TRY(hipStreamWaitEvent(c.aux_stream, c.aux_ev[0], 0));
+    if (int rc = hilbert_order(c, nact, c.xs, nullptr, c.walk_order, c.aux_stream, c.aux_tmp,
+                               c.aux_tmp_bytes))
+      return rc;
+    QV_TRY(hipEventRecord(c.aux_ev[1], c.aux_stream));
+    c.aux_live = true;
+  }
   // 5. nodes per sorted position, prefix sum -> DFS preorder offsets
   QV_TRY(hipMemsetAsync(c.nnode_of + nact, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_count_nodes, dim3(grid_of(nact)), dim3(kBlock), 0, st, nact, n, c.key0_s,
@@ -780,6 +798,8 @@ int tree_build(Ctx &c) {
   QV_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_moments_big, dim3(64), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
+  // the walk reads the order: the second stream joins here
+  if (int rc = join_aux(c)) return rc;
   return QV_OK;
 }
 
