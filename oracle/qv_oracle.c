@@ -20,6 +20,9 @@
  *                                 (bounds, occupancy, moments), golden
  *                                 velocities, theta->0 reduces to the pair sum
  *   pmotion AB ladder             golden trajectories, polynomial exactness
+ *   leaf sum (theta -> 0+)        the theta = 1e-9 walk (same leaf terms and
+ *                                 count) and a numpy brute force of the
+ *                                 tree's exclusion rule
  */
 #include "qv_oracle.h"
 
