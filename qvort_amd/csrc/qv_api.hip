@@ -79,6 +79,7 @@ int reserve(Ctx &c, int32_t n) {
   TRY(dalloc(c.walk_part, 3 * cap));
   TRY(dalloc(c.nnode_of, cap + 1));
   TRY(dalloc(c.node_off, cap + 1));
+  TRY(dalloc(c.sub_list, cap / 2 + 2));
   c.cap = (int32_t)cap;
   return QV_OK;
 }
@@ -103,7 +104,7 @@ int upload_vec(Ctx &c, const double *host, Vec3 v) {
 int velocity_stage(Ctx &c) {
   if (c.pcount == 0) return QV_ERR_STATE;
   TRY(hipEventRecord(c.ev[0], c.stream));
-  TRY(hipMemsetAsync(c.counters, 0, 4 * sizeof(int32_t), c.stream));
+  TRY(hipMemsetAsync(c.counters, 0, 8 * sizeof(int32_t), c.stream));
   TRY(hipMemsetAsync(c.evals, 0, 4 * sizeof(int64_t), c.stream));
   TRY(launch_ghost_lia(c));
   c.ghosts_valid = true;
@@ -322,7 +323,7 @@ int qv_destroy(qv_ctx *ctx) {
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
                 c->slot_s, c->walk_order, c->walk_part, c->nnode_of, c->node_off, c->sort_tmp,
-                c->counters, c->evals,
+                c->counters, c->evals, c->sub_list,
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,

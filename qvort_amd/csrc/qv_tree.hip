@@ -21,9 +21,21 @@
 // subtree (the skip pointer of a stackless DFS).
 //
 // Occupancy, cell bounds and the leaf/internal split are bit-identical to
-// the reference.  Moments are sums over the member range (parallel order):
-// cent = sum x / n, circ = sum (g - x), C = sum x (g - x)^T - cent circ^T
-// (tree.f90:154-179), equal to the reference's up to rounding.
+// the reference.  Moments (tree.f90:154-179): cent = sum x / n, circ =
+// sum ((circ + g) - x), C = sum x (g - x)^T - cent circ^T.  The reference
+// sums each node's members in slot order (a child's member list is a stable
+// filter of its parent's, and the root's is f(:) itself).  Every node of at
+// most 64 members does the same here, so its moments are the reference's
+// bit for bit: a wavefront takes a subtree of at most 64 members, sorts its
+// members by slot across the lanes and lets each lane fold one node's
+// members in that order.  Larger nodes (the top levels: about 1% of the
+// nodes at 1M points) are summed in chunks in parallel and differ from the
+// reference's sums in their last bits.  circ is a sum of segment vectors
+// that cancels over every closed loop inside a node, so its rounding is a
+// large share of the value, and the cell term of a node carries it to the
+// velocity: at 1M points, sums in key order moved the worst target's
+// velocity by 1.4e-12 relative, slot order for the nodes up to 64 members
+// leaves 1.2e-13.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -286,9 +298,14 @@ struct MomArgs {
   float *mp32;
   int32_t *big_list;   // big node ids (count > kSmallNode), in discovery order
   int32_t *big_base;   // first chunk of each big node
-  int32_t *big_count;  // [0] big nodes, [1] chunks
+  int32_t *big_count;  // [0] big nodes, [1] chunks, [2] subtrees
   int32_t *chunk_big;  // big-list index of each chunk
   double *partial;     // 15 sums per chunk
+  const int32_t *slots;  // slot of each sorted position
+  // roots of the subtrees whose nodes are summed in slot order: the nodes of
+  // 2..kSmallNode members whose parent is bigger (or the root, when it
+  // holds at most kSmallNode points); at most nact / 2 of them
+  int32_t *sub_list;
 };
 
 constexpr int kChunk = 4096;  // members per workgroup in the big-node sums
@@ -404,16 +421,27 @@ __device__ inline void finalize_node(const MomArgs &A, int64_t node, int n, doub
   for (int k = 9; k < 16; ++k) m32[k] = 0.f;
 }
 
-__device__ inline void add_point(const MomArgs &A, int t, double s[15]) {
-  const double x[3] = {A.xs[t], A.ys[t], A.zs[t]};
-  const double e[3] = {A.gxs[t] - x[0], A.gys[t] - x[1], A.gzs[t] - x[2]};
-  s[0] += x[0]; s[1] += x[1]; s[2] += x[2];
-  s[3] += e[0]; s[4] += e[1]; s[5] += e[2];
-  for (int m = 0; m < 3; ++m)
-    for (int k = 0; k < 3; ++k) s[6 + 3 * m + k] += x[m] * e[k];
+// one member's terms in new_tree's operation order (tree.f90:154-165):
+// cent + x, (circ + g) - x, C(:,k) + x (g_k - x_k)
+__device__ inline void add_member(const double x[3], const double g[3], double s[15]) {
+  s[0] = s[0] + x[0]; s[1] = s[1] + x[1]; s[2] = s[2] + x[2];
+  s[3] = (s[3] + g[0]) - x[0]; s[4] = (s[4] + g[1]) - x[1]; s[5] = (s[5] + g[2]) - x[2];
+  for (int k = 0; k < 3; ++k) {
+    const double e = g[k] - x[k];
+    for (int m = 0; m < 3; ++m) s[6 + 3 * m + k] = s[6 + 3 * m + k] + x[m] * e;
+  }
 }
 
-// one thread per small node; big nodes are queued for k_moments_big
+__device__ inline void add_point(const MomArgs &A, int t, double s[15]) {
+  const double x[3] = {A.xs[t], A.ys[t], A.zs[t]};
+  const double g[3] = {A.gxs[t], A.gys[t], A.gzs[t]};
+  add_member(x, g, s);
+}
+
+// one thread per node: leaves and empty records here; nodes of
+// 2..kSmallNode members are summed by k_moments_subtree, bigger ones are
+// queued for k_moments_chunks / k_moments_big and list their children of
+// 2..kSmallNode members as subtrees
 __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= A.nnodes) return;
@@ -427,6 +455,18 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
     A.big_list[q] = (int)node;
     A.big_base[q] = base;
     for (int k = 0; k < nch; ++k) A.chunk_big[base + k] = q;
+    // its children follow it in preorder: node + 1, then each child's skip
+    const int end = A.meta[node].x;
+    for (int ch = (int)node + 1; ch < end; ch = A.meta[ch].x) {
+      const int cn = A.count[ch];
+      if (cn >= 2 && cn <= kSmallNode) A.sub_list[atomicAdd(A.big_count + 2, 1)] = ch;
+    }
+    return;
+  }
+  if (n >= 2 || (is_root && n >= 1)) {
+    // summed in slot order by k_moments_subtree: the root itself is a
+    // subtree when it holds at most kSmallNode points
+    if (is_root) A.sub_list[atomicAdd(A.big_count + 2, 1)] = 0;
     return;
   }
   const int f = A.first[node];
@@ -452,9 +492,6 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
     for (int k = 3; k < 12; ++k) mp[k] = 0.;
     return;
   }
-  for (int k = 0; k < 15; ++k) s[k] = 0.;
-#pragma unroll 4
-  for (int t = f; t < f + n; ++t) add_point(A, t, s);
   if (n == 0) {
     NodeRec r;
     r.cx = r.cy = r.cz = 0.;
@@ -465,7 +502,75 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
     A.rec[node] = r;
     return;
   }
-  finalize_node(A, node, n, s, is_root);
+}
+
+// Slot-order moments of every node in a subtree of at most kSmallNode
+// members, one wavefront per subtree.  The subtree's members (a contiguous
+// range of sorted positions, in key order) are sorted by slot across the
+// lanes (bitonic sort on (slot, position)); lane r then holds the r-th
+// member in slot order and stages its position and ghost in LDS.  Each
+// lane takes one node of the subtree (its nodes are the preorder range
+// [root, skip)) and adds, in slot order, the members whose sorted position
+// lies in the node's range -- the members of its list in new_tree, in the
+// order new_tree meets them.
+constexpr int kSubWaves = kBlock / kWave;
+
+__global__ void __launch_bounds__(kBlock) k_moments_subtree(MomArgs A) {
+  __shared__ int32_t sq[kSubWaves][kWave];
+  __shared__ double sx[kSubWaves][6][kWave];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int nsub = A.big_count[2];
+  for (int w = blockIdx.x * kSubWaves + wid; w < nsub; w += gridDim.x * kSubWaves) {
+    const int f = A.sub_list[w];
+    const int first = f == 0 ? 0 : A.first[f];
+    const int n = f == 0 ? A.nact : A.count[f];  // 1..kSmallNode
+    // members by slot: key = slot (lanes past n sort last), value = the
+    // member's index in the subtree's key-order range
+    int key = lane < n ? A.slots[first + lane] : INT32_MAX;
+    int val = lane;
+    for (int k = 2; k <= kWave; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int pk = __shfl_xor(key, j), pv = __shfl_xor(val, j);
+        const bool up = (lane & k) == 0, low = (lane & j) == 0;
+        // the lower lane of a pair keeps the smaller key in an ascending
+        // run, the larger in a descending one
+        const bool take = (low == up) ? (pk < key) : (pk > key);
+        if (take) { key = pk; val = pv; }
+      }
+    }
+    if (lane < n) {
+      const int t = first + val;
+      sq[wid][lane] = val;
+      sx[wid][0][lane] = A.xs[t]; sx[wid][1][lane] = A.ys[t]; sx[wid][2][lane] = A.zs[t];
+      sx[wid][3][lane] = A.gxs[t]; sx[wid][4][lane] = A.gys[t]; sx[wid][5][lane] = A.gzs[t];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int end = A.meta[f].x;
+    for (int base = f; base < end; base += kWave) {
+      const int v = base + lane;
+      const bool own = v < end;
+      const bool root = v == 0;
+      const int cnt = !own ? 0 : (root ? A.nact : A.count[v]);
+      // leaves are written by k_moments_small
+      const bool sum = own && (cnt >= 2 || (root && cnt >= 1));
+      const int lo = sum ? (root ? 0 : A.first[v] - first) : 0;
+      const int hi = sum ? lo + cnt : 0;
+      double s[15];
+      for (int k = 0; k < 15; ++k) s[k] = 0.;
+      for (int r = 0; r < n; ++r) {
+        const int q = sq[wid][r];
+        if (q >= lo && q < hi) {
+          const double x[3] = {sx[wid][0][r], sx[wid][1][r], sx[wid][2][r]};
+          const double g[3] = {sx[wid][3][r], sx[wid][4][r], sx[wid][5][r]};
+          add_member(x, g, s);
+        }
+      }
+      if (sum) finalize_node(A, v, cnt, s, root);
+    }
+    // the next subtree overwrites this wavefront's LDS rows
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // one workgroup per chunk of a big node's member range; partial sums per
@@ -792,7 +897,13 @@ int tree_build(Ctx &c) {
   MA.chunk_big = c.big_buf + 2 * big_cap;
   MA.partial = c.partial;
   MA.big_count = c.counters + 2;
+  MA.slots = c.slot_s;
+  MA.sub_list = c.sub_list;
   hipLaunchKernelGGL(k_moments_small, dim3(grid_of(nnodes)), dim3(kBlock), 0, st, MA);
+  QV_TRY(hipGetLastError());
+  // one wavefront per subtree of >= 2 points: at most nact / 2 of them,
+  // about nact / 20 on random loops; waves past the list end return at once
+  hipLaunchKernelGGL(k_moments_subtree, dim3(nact / 64 + 1), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_moments_chunks, dim3(2048), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());

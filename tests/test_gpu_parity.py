@@ -2,8 +2,9 @@
 and against the oracle, through the C ABI.
 
 Bars (FP64 path): tree occupancy, cell bounds, leaf indices and the eval
-counter identical; direct (BS) sums bit-identical; tree velocities within
-1e-12 relative per point; AB trajectories within 1e-15 absolute.  FP32C
+counter identical; the moments of every node of at most 64 points
+identical; direct (BS) sums bit-identical; tree velocities within 1e-12
+relative per point; AB trajectories within 1e-15 absolute.  FP32C
 path: within 1e-5 relative per point of the FP64 reference.
 """
 import numpy as np
@@ -57,12 +58,19 @@ def test_tree_nodes_vs_golden(name):
         c.upload(g["x0"], g["infront"], g["behind"])
         c.velocity()
         t = c.tree_dump()
-    ref = g["tree"]
+    _check_tree(t, g["tree"], meta["params"]["box_size"])
+
+
+def _check_tree(t, ref, L):
     assert t.shape == ref.shape
     # level, pcount, leaf index, corner and width: bit-identical
     assert np.array_equal(t[:, 0:7], ref[:, 0:7])
-    # centroid, circulation and C: equal up to summation order
-    L = meta["params"]["box_size"]
+    # moments of every node of at most 64 points (summed in slot order, as
+    # new_tree sums them): bit-identical
+    small = (ref[:, 1] <= 64) & (ref[:, 0] > 0)
+    assert np.array_equal(t[small, 7:22], ref[small, 7:22])
+    _say("nodes", len(ref), "bit-identical", int(small.sum()))
+    # bigger nodes (summed in chunks): equal up to summation order
     assert np.max(np.abs(t[:, 7:10] - ref[:, 7:10])) <= 1e-15 * L
     circ_scale = np.max(np.abs(ref[:, 10:13])) + 1e-300
     assert np.max(np.abs(t[:, 10:13] - ref[:, 10:13])) <= 1e-13 * circ_scale
@@ -121,10 +129,12 @@ def test_fp64_tree_vs_oracle_20k(theta, boundary):
     with _ctx(gpu_params(kw)) as c:
         c.upload_filaments(fil)
         u, uli, ubs, ev = c.velocity()
+        t = c.tree_dump()
     assert ev == ev_o
     e = rel_err(u, u_o)
     _say("max", e.max())
     assert np.max(e) <= 1e-12
+    _check_tree(t, port.tree_dump(port.make_params(**kw), fil), 0.1)
     with _ctx(gpu_params(kw, precision="fp32c")) as c:
         c.upload_filaments(fil)
         u32, _, _, ev32 = c.velocity()
