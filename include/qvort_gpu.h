@@ -66,7 +66,9 @@ enum qv_status {
   QV_ERR_NO_DEVICE = 6,  /* no HIP device visible */
   QV_ERR_LINE_EMPTY = 7, /* no active point: the fatal_error of line.f90:27
                             ("vortex line length is 0, run over") */
-  QV_ERR_CAPACITY = 8    /* pinsert needs more slots than the arrays hold */
+  QV_ERR_CAPACITY = 8,   /* pinsert needs more slots than the arrays hold */
+  QV_ERR_SCRATCH = 9     /* qv_vjp_device_noalloc: more slots than its static
+                            scratch holds (65536) */
 };
 
 /* the run.in parameters of the hot path (cdata.f90:196-416) */
@@ -205,6 +207,18 @@ int qv_set_stream(qv_ctx *ctx, void *hip_stream);
 int qv_velocity_device(qv_ctx *ctx, const double *d_x, double *d_u);
 int qv_vjp_device(qv_ctx *ctx, const double *d_x, const double *d_w, double *d_xbar);
 int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_jv);
+/* qv_vjp_device without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * scratch of the adjoint kernels is a static device array of the library
+ * (30 doubles per slot, at most 65536 slots: 15.7 MB per device, held until
+ * the process ends) instead of buffers qv_vjp_device allocates on its first
+ * call.  Same arguments, same result bit for bit.  More than 65536 slots:
+ * QV_ERR_SCRATCH.  Every call on one device shares that device's array:
+ * calls that overlap on one GPU (two contexts on different streams, or two
+ * threads) race. */
+int qv_vjp_device_noalloc(qv_ctx *ctx, const double *d_x, const double *d_w,
+                          double *d_xbar);
 
 /* ---- ensembles of independent systems.  No counterpart in the reference,
  * which runs one system per process (run.f90).  Many filament systems of

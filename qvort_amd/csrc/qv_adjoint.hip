@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_jvp(AdjArgs A, Shells S) {
 
 inline int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
-AdjArgs make_args(Ctx &c, Vec3 w, Vec3 r, int mixed) {
+AdjArgs make_args(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch) {
   AdjArgs A;
   A.n = c.pcount;
   A.order = c.p.deriv_order;
@@ -408,15 +408,15 @@ AdjArgs make_args(Ctx &c, Vec3 w, Vec3 r, int mixed) {
   A.JX = c.gbb.x; A.JY = c.gbb.y; A.JZ = c.gbb.z;
   A.inf = c.infront; A.beh = c.behind;
   A.wx = w.x; A.wy = w.y; A.wz = w.z;
-  A.out = c.adj;
+  A.out = scratch;
   A.rx = r.x; A.ry = r.y; A.rz = r.z;
   return A;
 }
 
 }  // namespace
 
-hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed) {
-  AdjArgs A = make_args(c, w, r, mixed);
+hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch) {
+  AdjArgs A = make_args(c, w, r, mixed, scratch);
   Shells S = make_shells(c.p);
   const int g = grid(c.pcount);
   hipLaunchKernelGGL(k_adj_target, dim3(g), dim3(kBlock), 0, c.stream, A, S);
@@ -427,7 +427,7 @@ hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed) {
 }
 
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r) {
-  AdjArgs A = make_args(c, v, r, 0);
+  AdjArgs A = make_args(c, v, r, 0, nullptr);
   Shells S = make_shells(c.p);
   hipLaunchKernelGGL(k_jvp, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, A, S);
   return hipGetLastError();

@@ -3,6 +3,7 @@
 // reference routine each entry point replaces: include/qvort_gpu.h.
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -10,6 +11,38 @@
 
 namespace qv {
 namespace {
+
+// Scratch of qv_vjp_device_noalloc: per slot the cotangent and the result
+// (3 + 3 doubles, SoA) and the adjoint kernels' 24 partial sums, for up to
+// kNoAllocSlots slots (15.7 MB).  Each device the library runs on holds its
+// own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, not a device address: its address on a device comes
+// from hipGetSymbolAddress with that device current (scratch_setup, called
+// by qv_create) and only that address is passed to kernels.
+constexpr int32_t kNoAllocSlots = 65536;
+constexpr size_t kScratchDoubles = (size_t)30 * kNoAllocSlots;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double *scratch_dev[kMaxDevices] = {};
+
+// the address of `scratch` on `device` (current), looked up once per device
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return QV_ERR_HIP;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_dev[device]) return QV_OK;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return QV_ERR_HIP;
+  scratch_dev[device] = static_cast<double *>(ptr);
+  return QV_OK;
+}
+
+double *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return scratch_dev[device];
+}
 
 int check_params(const qv_params *p) {
   if (!p) return QV_ERR_ARG;
@@ -236,7 +269,7 @@ int derivative(Ctx &c, const double *d_in, double *d_out, bool vjp) {
   if (rc) return rc;
   TRY(launch_ghost_lia(c));
   TRY(launch_aos_to_soa(c, d_in, c.aw));
-  if (vjp) TRY(launch_vjp(c, c.aw, c.ar, c.p.precision == QV_FP32C ? 1 : 0));
+  if (vjp) TRY(launch_vjp(c, c.aw, c.ar, c.p.precision == QV_FP32C ? 1 : 0, c.adj));
   else TRY(launch_jvp(c, c.aw, c.ar));
   TRY(launch_soa_to_aos(c, c.ar, d_out));
   return QV_OK;
@@ -279,6 +312,11 @@ int qv_create(const qv_params *params, qv_ctx **out) {
   if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return QV_ERR_NO_DEVICE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return QV_ERR_HIP;
+  // the no-allocation entry point's scratch on this device
+  rc = scratch_setup(dev);
+  if (rc) return rc;
   Ctx *c = new (std::nothrow) Ctx();
   if (!c) return QV_ERR_HIP;
   c->p = *params;
@@ -651,6 +689,28 @@ int qv_vjp_device(qv_ctx *ctx, const double *d_x, const double *d_w, double *d_x
   return derivative(*c, d_w, d_xbar, true);
 }
 
+int qv_vjp_device_noalloc(qv_ctx *ctx, const double *d_x, const double *d_w,
+                          double *d_xbar) {
+  if (!ctx || !d_w || !d_xbar) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  if (c->pcount > kNoAllocSlots) return QV_ERR_SCRATCH;
+  if (c->p.velocity != QV_VEL_BS || !c->links_consistent) return QV_ERR_ARG;
+  double *base = scratch_of(c->device);
+  if (!base) return QV_ERR_STATE;
+  const size_t n = (size_t)c->pcount;
+  const Vec3 w{base, base + n, base + 2 * n};
+  const Vec3 r{base + 3 * n, base + 4 * n, base + 5 * n};
+  double *adj = base + 6 * n;  // 24 n doubles
+  int rc = set_positions_device(*c, d_x);
+  if (rc) return rc;
+  TRY(launch_ghost_lia(*c));
+  TRY(launch_aos_to_soa(*c, d_w, w));
+  TRY(launch_vjp(*c, w, r, c->p.precision == QV_FP32C ? 1 : 0, adj));
+  TRY(launch_soa_to_aos(*c, r, d_xbar));
+  return QV_OK;
+}
+
 int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_jv) {
   if (!ctx || !d_v || !d_jv) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -679,6 +739,8 @@ const char *qv_status_string(int status) {
     case QV_ERR_NO_DEVICE: return "no HIP device";
     case QV_ERR_LINE_EMPTY: return "line.mod:pinsert: vortex line length is 0, run over";
     case QV_ERR_CAPACITY: return "line operations need more slots than the arrays hold";
+    case QV_ERR_SCRATCH:
+      return "qv_vjp_device_noalloc: more than 65536 slots, the size of its static scratch";
     default: return "unknown status";
   }
 }

@@ -187,7 +187,8 @@ int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order, h
 int join_aux(Ctx &c);
 int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
-hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed);
+// scratch: 24 doubles per slot, every one written before it is read
+hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 hipError_t launch_closest(Ctx &c, bool tree);
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,

@@ -29,7 +29,8 @@ module qv_gpu
   integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
                                        QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
                                        QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6, &
-                                       QV_ERR_LINE_EMPTY = 7, QV_ERR_CAPACITY = 8
+                                       QV_ERR_LINE_EMPTY = 7, QV_ERR_CAPACITY = 8, &
+                                       QV_ERR_SCRATCH = 9
 
   !> run.in parameters of the hot path (cdata.f90:196-416)
   type, bind(C), public :: qv_params

@@ -27,13 +27,14 @@ BC = {"open": 0, "periodic": 1, "openx": 2, "openxy": 3}
 PREC = {"fp64": 0, "fp32c": 1}
 
 QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
-    QV_ERR_NO_DEVICE, QV_ERR_LINE_EMPTY, QV_ERR_CAPACITY = range(9)
+    QV_ERR_NO_DEVICE, QV_ERR_LINE_EMPTY, QV_ERR_CAPACITY, QV_ERR_SCRATCH = range(10)
 QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE, QV_PROBE_EMPTY_CORE = 1, 2, 4
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
            "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic", "qv_step", "qv_download",
            "qv_tree_dump", "qv_closest", "qv_probe_velocity", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
-           "qv_vjp_device", "qv_jvp_device", "qv_get_timings", "qv_batch_create",
+           "qv_vjp_device", "qv_jvp_device", "qv_vjp_device_noalloc", "qv_get_timings",
+           "qv_batch_create",
            "qv_batch_destroy", "qv_batch_upload", "qv_batch_velocity", "qv_batch_step",
            "qv_batch_download", "qv_batch_tune", "qv_batch_set_variant", "qv_batch_info",
            "qv_download_ghosts", "qv_line_pinsert", "qv_line_precon", "qv_line_premove",
@@ -137,6 +138,7 @@ def lib():
         L.qv_velocity_device.argtypes = [vp, vp, vp]
         L.qv_vjp_device.argtypes = [vp, vp, vp, vp]
         L.qv_jvp_device.argtypes = [vp, vp, vp, vp]
+        L.qv_vjp_device_noalloc.argtypes = [vp, vp, vp, vp]
         L.qv_batch_create.argtypes = [P(QvParams), C.c_int32, i32, P(vp)]
         L.qv_batch_destroy.argtypes = [vp]
         L.qv_batch_upload.argtypes = [vp, d, i32, i32]
@@ -344,6 +346,9 @@ class Context:
 
     def vjp_device(self, d_x: int | None, d_w: int, d_out: int) -> None:
         _check(lib().qv_vjp_device(self._h, d_x, d_w, d_out), "qv_vjp_device")
+
+    def vjp_device_noalloc(self, d_x: int | None, d_w: int, d_out: int) -> None:
+        _check(lib().qv_vjp_device_noalloc(self._h, d_x, d_w, d_out), "qv_vjp_device_noalloc")
 
     def jvp_device(self, d_x: int | None, d_v: int, d_out: int) -> None:
         _check(lib().qv_jvp_device(self._h, d_x, d_v, d_out), "qv_jvp_device")
