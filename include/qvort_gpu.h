@@ -49,6 +49,12 @@ enum qv_velocity { QV_VEL_BS = 1, QV_VEL_TREE = 2 };
 enum qv_boundary { QV_BC_OPEN = 0, QV_BC_PERIODIC = 1, QV_BC_OPENX = 2, QV_BC_OPENXY = 3 };
 /* arithmetic of the nonlocal sum */
 enum qv_precision { QV_FP64 = 0, QV_FP32C = 1 };
+/* qv_params.flags: run.in switches of pmotion.  QV_FLAG_REMOVE_MEAN_POS:
+ * remove_mean_pos T, the mean x and y of the active points subtracted from
+ * every slot after each AB update (timestep.f90:86-92).  (pmotion's
+ * dt_adapt, timestep.f90:62-85, has no run.in label in the reference,
+ * cdata.f90:420-807, so it is always off there and has no flag here.) */
+enum qv_param_flags { QV_FLAG_REMOVE_MEAN_POS = 1 };
 /* flags of qv_probe_velocity */
 enum qv_probe_flags { QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2, QV_PROBE_EMPTY_CORE = 4 };
 
@@ -84,7 +90,7 @@ typedef struct qv_params {
   int32_t tree_extra_correction; /* `tree_extra_correction` (reference default 1) */
   int32_t deriv_order;           /* `deriv_order`: 2 ('second') or 4 ('fourth') */
   int32_t precision;             /* enum qv_precision */
-  int32_t reserved;              /* must be 0 */
+  int32_t flags;                 /* enum qv_param_flags; other bits must be 0 */
 } qv_params;
 
 /* per-stage timings of the last qv_velocity (device events, milliseconds) */
@@ -117,15 +123,32 @@ int qv_upload_history(qv_ctx *ctx, const double *u1, const double *u2,
  * NULL): eval_counter of the stage (Tree), 64-bit. */
 int qv_velocity(qv_ctx *ctx, double *u, double *u_s_li, double *u_s_bs,
                 int64_t *evals);
-/* AB ladder + history shift with the velocity of the last qv_velocity */
+/* AB ladder + history shift with the velocity of the last qv_velocity
+ * (then remove_mean_pos with QV_FLAG_REMOVE_MEAN_POS) */
 int qv_ab_step(qv_ctx *ctx);
-/* periodic wrap of the device positions */
+/* the boundary step of run.f90:82-90 on the device state: enforce_periodic
+ * (periodic.f90:188-213) for periodic; for openx / openxy
+ * enforce_periodic_yz / _z (periodic.f90:217-276), which wrap the periodic
+ * axes and remove every loop with a point beyond an open face
+ * (boundary_loop_remove, periodic.f90:306-327, slots cleared as
+ * clear_particle clears them).  The device checks for such points; only
+ * when one exists is the state brought to the host, where the removal runs
+ * in the reference's slot order (its length counter depends on it), and
+ * sent back.  open: nothing. */
 int qv_enforce_periodic(qv_ctx *ctx);
-/* nsteps of: velocity, AB step, wrap -- all on the device */
+/* boundary_loop_remove_count and _length (cdata.f90:316-317) summed over
+ * the context's qv_enforce_periodic / qv_step calls */
+int qv_boundary_removed(qv_ctx *ctx, int64_t *count, double *length);
+/* nsteps of: velocity, AB step, boundary step -- on the device (openx /
+ * openxy: one device-to-host flag read per step) */
 int qv_step(qv_ctx *ctx, int32_t nsteps);
 /* device state back to the host (NULL = skip) */
 int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2,
                 double *u3);
+/* the device links f(:)%infront / %behind (NULL = skip): they change on the
+ * device only when qv_enforce_periodic / qv_step remove loops at an open
+ * face (qv_boundary_removed counts them) */
+int qv_download_links(qv_ctx *ctx, int32_t *infront, int32_t *behind);
 
 /* the tree of the last Tree-mode qv_velocity, every node (empty children
  * included) in DFS order, 22 doubles per node: level, pcount, leaf index j

@@ -22,15 +22,18 @@ class QvoParams(C.Structure):
     _fields_ = [("quant_circ", C.c_double), ("corea", C.c_double), ("box_size", C.c_double),
                 ("delta", C.c_double), ("tree_theta", C.c_double), ("dt", C.c_double),
                 ("velocity", C.c_int32), ("boundary", C.c_int32),
-                ("tree_extra_correction", C.c_int32), ("deriv_order", C.c_int32)]
+                ("tree_extra_correction", C.c_int32), ("deriv_order", C.c_int32),
+                ("remove_mean_pos", C.c_int32)]
 
 
 def make_params(*, quant_circ=9.97e-4, corea=8.244023e-9, box_size=0.1, delta=5e-4,
                 tree_theta=0.0, dt=1e-6, velocity="Tree", boundary="periodic",
-                tree_extra_correction=True, deriv_order=2) -> QvoParams:
+                tree_extra_correction=True, deriv_order=2,
+                remove_mean_pos=False) -> QvoParams:
     return QvoParams(quant_circ, corea, box_size, delta, tree_theta, dt,
                      VEL_TREE if velocity == "Tree" else VEL_BS, BC[boundary],
-                     1 if tree_extra_correction else 0, int(deriv_order))
+                     1 if tree_extra_correction else 0, int(deriv_order),
+                     1 if remove_mean_pos else 0)
 
 
 _lib = None
@@ -62,12 +65,15 @@ def lib():
         L.qvo_ghostp.restype = C.c_int
         L.qvo_ab_update.argtypes = [P(QvoParams), C.c_int32, d, i32, d, d, d, d]
         L.qvo_ab_update.restype = C.c_int
-        L.qvo_run.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, C.c_int32, d, C.c_int32]
+        L.qvo_run.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, d, C.c_int32, d, C.c_int32,
+                              i64, d]
         L.qvo_run.restype = C.c_int
         L.qvo_segment_velocity.argtypes = [P(QvoParams), d, d, d, d, C.c_int32, d]
         L.qvo_segment_velocity.restype = C.c_int
         L.qvo_enforce_periodic.argtypes = [P(QvoParams), C.c_int32, d, i32]
         L.qvo_enforce_periodic.restype = C.c_int
+        L.qvo_enforce_boundary.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, d, d, i64, d]
+        L.qvo_enforce_boundary.restype = C.c_int
         L.qvo_closest_tree.argtypes = [P(QvoParams), C.c_int32, d, i32, i32, C.c_int32, d, i32,
                                        i32, i32, d, C.c_int32]
         L.qvo_closest_tree.restype = C.c_int
@@ -168,19 +174,40 @@ def ab_update(params: QvoParams, x, infront, u, u1, u2, u3):
     return x, u1, u2, u3
 
 
-def run(params: QvoParams, fil, nsteps: int, u1=None, u2=None, u3=None, nthreads: int = 0):
+def run(params: QvoParams, fil, nsteps: int, u1=None, u2=None, u3=None, nthreads: int = 0,
+        topology: bool = False):
+    """nsteps of velocity, AB ladder (and remove_mean_pos) and the boundary
+    step; returns (x, u1, u2, u3, u of the first step), with topology=True
+    also the links after the run and boundary_loop_remove's count and
+    length."""
     x, inf, beh = _prep(fil)
-    x = x.copy()
+    x, inf, beh = x.copy(), inf.copy(), beh.copy()
     n = x.shape[0]
     u1 = np.zeros((n, 3)) if u1 is None else np.array(u1, dtype=np.float64, order="C")
     u2 = np.zeros((n, 3)) if u2 is None else np.array(u2, dtype=np.float64, order="C")
     u3 = np.zeros((n, 3)) if u3 is None else np.array(u3, dtype=np.float64, order="C")
     uf = np.zeros((n, 3))
+    cnt, length = C.c_int64(0), C.c_double(0.0)
     rc = lib().qvo_run(C.byref(params), n, _d(x), _i(inf), _i(beh), _d(u1), _d(u2), _d(u3),
-                       nsteps, _d(uf), nthreads)
+                       nsteps, _d(uf), nthreads, C.byref(cnt), C.byref(length))
     if rc != 0:
         raise RuntimeError(f"qvo_run failed ({rc})")
+    if topology:
+        return x, u1, u2, u3, uf, inf, beh, int(cnt.value), float(length.value)
     return x, u1, u2, u3, uf
+
+
+def enforce_boundary(params: QvoParams, x, infront, behind, u1, u2):
+    """The boundary step on copies: (x, infront, behind, u1, u2, count, length)."""
+    x = np.array(x, dtype=np.float64, order="C")
+    inf = np.array(infront, dtype=np.int32, order="C")
+    beh = np.array(behind, dtype=np.int32, order="C")
+    u1 = np.array(u1, dtype=np.float64, order="C")
+    u2 = np.array(u2, dtype=np.float64, order="C")
+    cnt, length = C.c_int64(0), C.c_double(0.0)
+    lib().qvo_enforce_boundary(C.byref(params), x.shape[0], _d(x), _i(inf), _i(beh), _d(u1),
+                               _d(u2), C.byref(cnt), C.byref(length))
+    return x, inf, beh, u1, u2, int(cnt.value), float(length.value)
 
 
 def segment_velocity(params: QvoParams, xi, xj, gj, shift=(0.0, 0.0, 0.0), form: int = 0):
@@ -230,9 +257,9 @@ def probe_velocity(params: QvoParams, build, current, ghosti, probes, ref_scalin
 
 
 def run_rk3(params: QvoParams, fil, nsteps: int, u1=None, nthreads: int = 0):
-    """nsteps of pmotion_RK3 + wrap; returns (x, u1, u_last)."""
+    """nsteps of pmotion_RK3 + the boundary step; returns (x, u1, u_last)."""
     x, inf, beh = _prep(fil)
-    x = x.copy()
+    x, inf, beh = x.copy(), inf.copy(), beh.copy()
     n = x.shape[0]
     u1 = np.zeros((n, 3)) if u1 is None else np.array(u1, dtype=np.float64, order="C")
     ul = np.zeros((n, 3))

@@ -752,6 +752,23 @@ int qvo_ab_update(const qvo_params *p, int32_t pcount, double *x,
       a1[k] = ui[k];
     }
   }
+  if (p->remove_mean_pos) {
+    /* timestep.f90:86-92: the mean x and y of the active points (SUM with a
+     * mask, a sequential sum in slot order), subtracted from every slot */
+    double sx = 0., sy = 0.;
+    int32_t n = 0;
+    for (int i = 0; i < pcount; ++i) {
+      if (infront[i] <= 0) continue;
+      sx = sx + x[3 * i];
+      sy = sy + x[3 * i + 1];
+      ++n;
+    }
+    const double mx = sx / n, my = sy / n;
+    for (int i = 0; i < pcount; ++i) {
+      x[3 * i] = x[3 * i] - mx;
+      x[3 * i + 1] = x[3 * i + 1] - my;
+    }
+  }
   return 0;
 }
 
@@ -771,6 +788,75 @@ int qvo_enforce_periodic(const qvo_params *p, int32_t pcount, double *x,
       else if (x[3 * i + k] < (-L / 2.)) x[3 * i + k] = x[3 * i + k] + L;
     }
   }
+  return 0;
+}
+
+/* distf (general.f90:27-34) */
+static double distf(const double *x, int i, int j) {
+  return sqrt((sq(x[3 * i] - x[3 * j]) + sq(x[3 * i + 1] - x[3 * j + 1])) +
+              sq(x[3 * i + 2] - x[3 * j + 2]));
+}
+
+/* boundary_loop_remove (periodic.f90:306-327): walk the loop from
+ * `particle`, adding each segment's length but the last one's, clearing
+ * every slot (clear_particle, general.f90:11-22) */
+static void boundary_loop_remove(int32_t pcount, double *x, int32_t *infront, int32_t *behind,
+                                 double *u1, double *u2, int particle, int64_t *count,
+                                 double *length) {
+  int next = particle;
+  *count += 1;
+  for (int it = 0; it < pcount; ++it) {
+    const int store_next = infront[next] - 1;
+    if (store_next < 0) break; /* not a closed loop: the reference reads f(0) */
+    if (store_next != particle) *length = *length + distf(x, next, store_next);
+    for (int k = 0; k < 3; ++k) {
+      x[3 * next + k] = 0.;
+      u1[3 * next + k] = 0.;
+      u2[3 * next + k] = 0.;
+    }
+    infront[next] = 0;
+    behind[next] = 0;
+    next = store_next;
+    if (next == particle) break;
+  }
+}
+
+int qvo_enforce_boundary(const qvo_params *p, int32_t pcount, double *x, int32_t *infront,
+                         int32_t *behind, double *u1, double *u2, int64_t *count,
+                         double *length) {
+  int64_t cnt = 0;
+  double len = 0.;
+  if (count) cnt = *count;
+  if (length) len = *length;
+  const double L = p->box_size;
+  for (int i = 0; i < pcount; ++i) {
+    if (infront[i] == 0) continue;
+    double *xi = x + 3 * i;
+    if (p->boundary == QVO_BC_PERIODIC) {
+      for (int k = 0; k < 3; ++k) {
+        if (xi[k] > (L / 2.)) xi[k] = xi[k] - L;
+        else if (xi[k] < (-L / 2.)) xi[k] = xi[k] + L;
+      }
+    } else if (p->boundary == QVO_BC_OPENX) {
+      /* enforce_periodic_yz (periodic.f90:217-243): wrap y and z, then x */
+      for (int k = 1; k < 3; ++k) {
+        if (xi[k] > (L / 2.)) xi[k] = xi[k] - L;
+        else if (xi[k] < (-L / 2.)) xi[k] = xi[k] + L;
+      }
+      if (xi[0] > (L / 2.) || xi[0] < (-L / 2.))
+        boundary_loop_remove(pcount, x, infront, behind, u1, u2, i, &cnt, &len);
+    } else if (p->boundary == QVO_BC_OPENXY) {
+      /* enforce_periodic_z (periodic.f90:247-272): x, then y on the
+       * (possibly cleared) point, then wrap z */
+      for (int k = 0; k < 2; ++k)
+        if (xi[k] > (L / 2.) || xi[k] < (-L / 2.))
+          boundary_loop_remove(pcount, x, infront, behind, u1, u2, i, &cnt, &len);
+      if (xi[2] > (L / 2.)) xi[2] = xi[2] - L;
+      else if (xi[2] < (-L / 2.)) xi[2] = xi[2] + L;
+    }
+  }
+  if (count) *count = cnt;
+  if (length) *length = len;
   return 0;
 }
 
@@ -805,9 +891,9 @@ int qvo_segment_velocity(const qvo_params *p, const double *xi, const double *xj
 }
 
 int qvo_run(const qvo_params *p, int32_t pcount, double *x,
-            const int32_t *infront, const int32_t *behind, double *u1,
+            int32_t *infront, int32_t *behind, double *u1,
             double *u2, double *u3, int32_t nsteps, double *u_first,
-            int32_t nthreads) {
+            int32_t nthreads, int64_t *bcount, double *blength) {
   double *u = (double *)malloc((size_t)3 * (size_t)pcount * sizeof(double));
   if (!u) return -1;
   for (int step = 0; step < nsteps; ++step) {
@@ -815,14 +901,14 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
     if (rc) { free(u); return rc; }
     if (step == 0 && u_first) memcpy(u_first, u, (size_t)3 * pcount * sizeof(double));
     qvo_ab_update(p, pcount, x, infront, u, u1, u2, u3);
-    qvo_enforce_periodic(p, pcount, x, infront);
+    qvo_enforce_boundary(p, pcount, x, infront, behind, u1, u2, bcount, blength);
   }
   free(u);
   return 0;
 }
 
 int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
-                const int32_t *infront, const int32_t *behind, double *u1,
+                int32_t *infront, int32_t *behind, double *u1,
                 int32_t nsteps, double *u_last, int32_t nthreads) {
   double *u = (double *)malloc((size_t)3 * (size_t)pcount * sizeof(double));
   if (!u) return -1;
@@ -849,7 +935,11 @@ int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
         }
       }
     }
-    qvo_enforce_periodic(p, pcount, x, infront);
+    /* the boundary step; u2 is not part of the Runge-Kutta state */
+    double *u2 = (double *)calloc((size_t)3 * (size_t)pcount, sizeof(double));
+    if (!u2) { free(u); return -1; }
+    qvo_enforce_boundary(p, pcount, x, infront, behind, u1, u2, NULL, NULL);
+    free(u2);
   }
   if (u_last) memcpy(u_last, u, (size_t)3 * pcount * sizeof(double));
   free(u);

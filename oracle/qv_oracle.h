@@ -31,6 +31,7 @@ typedef struct qvo_params {
   int32_t boundary;    /* QVO_BC_*     initial.f90:104-136 */
   int32_t tree_extra_correction; /*    cdata.f90:333 */
   int32_t deriv_order; /* 2 or 4       cdata.f90:218 */
+  int32_t remove_mean_pos; /* 0 or 1   cdata.f90:210, pmotion timestep.f90:86-92 */
 } qvo_params;
 
 /* node record written by qvo_tree_dump: 22 doubles per node, DFS order,
@@ -72,6 +73,16 @@ int qvo_ab_update(const qvo_params *p, int32_t pcount, double *x,
 /* wrap after a step: enforce_periodic / _yz / _z (periodic.f90:188-276) */
 int qvo_enforce_periodic(const qvo_params *p, int32_t pcount, double *x,
                          const int32_t *infront);
+/* the boundary step of run.f90:82-90: enforce_periodic (periodic), or
+ * enforce_periodic_yz (openx) / enforce_periodic_z (openxy), which wrap the
+ * periodic axes and remove every loop with a point beyond an open face
+ * (boundary_loop_remove, periodic.f90:306-327), slot by slot in the
+ * reference's order; removed slots are cleared as clear_particle clears them
+ * (x, u1, u2, links; u3 kept).  count / length (may be NULL) accumulate
+ * boundary_loop_remove_count / _length. */
+int qvo_enforce_boundary(const qvo_params *p, int32_t pcount, double *x, int32_t *infront,
+                         int32_t *behind, double *u1, double *u2, int64_t *count,
+                         double *length);
 
 /* one straight segment x_j -> g_j (+shift) seen from x_i; form 0 is the
  * biot_savart pair formula, form 1 the tree leaf formula */
@@ -83,9 +94,9 @@ int qvo_segment_velocity(const qvo_params *p, const double *xi, const double *xj
  * (run.f90:41-54, 75-107): ghostp, tree, velocity, AB, ghostp, wrap.
  * x/u1/u2/u3 updated in place; u_first receives the first step's velocity. */
 int qvo_run(const qvo_params *p, int32_t pcount, double *x,
-            const int32_t *infront, const int32_t *behind,
+            int32_t *infront, int32_t *behind,
             double *u1, double *u2, double *u3, int32_t nsteps,
-            double *u_first, int32_t nthreads);
+            double *u_first, int32_t nthreads, int64_t *bcount, double *blength);
 
 /* nsteps of the main loop with the Runge-Kutta integrator (runge_kutta T,
  * run.f90:50-52): pmotion_RK3 (timestep.f90:95-176) -- three stages of
@@ -94,7 +105,7 @@ int qvo_run(const qvo_params *p, int32_t pcount, double *x,
  * x += u1/2 -- then the wrap.  u1 is the Runge-Kutta storage, in/out;
  * u_last receives the last stage's velocity of the last step. */
 int qvo_run_rk3(const qvo_params *p, int32_t pcount, double *x,
-                const int32_t *infront, const int32_t *behind, double *u1,
+                int32_t *infront, int32_t *behind, double *u1,
                 int32_t nsteps, double *u_last, int32_t nthreads);
 
 /* theta -> 0+ limit of the Tree velocity (every leaf with tree_walk's

@@ -54,6 +54,8 @@
 !>   out.bin    int32 pcount; int64 eval_counter (first step)
 !>              real64 u, u_s_LI, u_s_BS, ghosti, ghostb   (3,pcount each, first step)
 !>              real64 x, u1, u2, u3                        (3,pcount each, after nsteps)
+!>              (flags bit 9) int32 infront, behind (pcount each, after nsteps);
+!>              int32 boundary_loop_remove_count; real64 boundary_loop_remove_length
 !>   tree.bin   (flags bit 0) every node of the first step's tree in DFS order
 !>   timing.txt (flags bit 1) build/walk wall times of a target sample
 !>   closest.bin (flags bit 2) int32 pcount after pinsert;
@@ -281,6 +283,14 @@ program qv_ref_driver
   do i = 1, pcount
     write(43) f(i)%u3
   end do
+  if (btest(flags, 9)) then
+    ! the links after the run and the counters of boundary_loop_remove
+    ! (periodic.f90:306-327), for runs whose loops leave through an open face
+    write(43) (f(i)%infront, i = 1, pcount)
+    write(43) (f(i)%behind, i = 1, pcount)
+    write(43) boundary_loop_remove_count
+    write(43) boundary_loop_remove_length
+  end if
   close(43)
 
 contains

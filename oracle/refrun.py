@@ -26,7 +26,7 @@ def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_siz
                 tree_extra_correction=True, deriv_order="second", nsteps=1,
                 runge_kutta=False, shots=1000000, tree_print=False,
                 recon_shots=1000000, mesh_size=0, mesh_shots=100,
-                mesh_print_delay=0) -> str:
+                mesh_print_delay=0, remove_mean_pos=False) -> str:
     lines = [f"runge_kutta {'T' if runge_kutta else 'F'}",
         f"nsteps {nsteps}", f"shots {shots}", f"recon_shots {recon_shots}", f"dt {dt!r}",
         f"tree_print {'T' if tree_print else 'F'}",
@@ -37,6 +37,7 @@ def run_in_text(*, velocity="Tree", tree_theta=0.4, boundary="periodic", box_siz
         f"deriv_order {deriv_order}", "initf single_loop", "pcount 16",
         f"mesh_size {mesh_size}", f"mesh_shots {mesh_shots}",
         f"mesh_print_delay {mesh_print_delay}",
+        f"remove_mean_pos {'T' if remove_mean_pos else 'F'}",
     ]
     return "\n".join(lines) + "\n"
 
@@ -55,7 +56,7 @@ def _write_state(path, fil, nsteps, flags, nsample, u1=None, u2=None, u3=None):
 
 def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None,
         u1=None, u2=None, u3=None, timeout=3600, dump_var=False, restore_var=None,
-        **params) -> dict:
+        topology=False, **params) -> dict:
     """Run the reference driver; returns a dict of numpy arrays.
 
     dump_var: also return the bytes of data/var.dat written after the run with
@@ -68,7 +69,7 @@ def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None
     if not available():
         raise FileNotFoundError(DRIVER)
     flags = ((1 if dump_tree else 0) | (2 if timing else 0) | (64 if dump_var else 0)
-             | (128 if restore_var is not None else 0))
+             | (128 if restore_var is not None else 0) | (512 if topology else 0))
     env = dict(os.environ)
     if threads is not None:
         env["OMP_NUM_THREADS"] = str(threads)
@@ -115,9 +116,17 @@ def run(fil, *, nsteps=1, dump_tree=False, timing=False, nsample=0, threads=None
         n = int(np.frombuffer(raw, dtype="<i4", count=1)[0])
         assert restore_var is not None or n == fil.pcount
         out["evals"] = int(np.frombuffer(raw, dtype="<i8", count=1, offset=4)[0])
-        arr = np.frombuffer(raw, dtype="<f8", offset=12).reshape(9, n, 3)
+        arr = np.frombuffer(raw, dtype="<f8", count=27 * n, offset=12).reshape(9, n, 3)
         for k, name in enumerate(["u", "u_li", "u_bs", "ghosti", "ghostb", "x", "u1", "u2", "u3"]):
             out[name] = arr[k].copy()
+        if topology:
+            off = 12 + 8 * 27 * n
+            links = np.frombuffer(raw, dtype="<i4", count=2 * n + 1, offset=off)
+            out["infront_final"] = links[:n].copy()
+            out["behind_final"] = links[n:2 * n].copy()
+            out["boundary_count"] = int(links[2 * n])
+            out["boundary_length"] = float(np.frombuffer(raw, dtype="<f8", count=1,
+                                                         offset=off + 4 * (2 * n + 1))[0])
         if dump_tree:
             out["tree"] = parse_tree(open(os.path.join(wd, "tree.bin"), "rb").read())
         return out

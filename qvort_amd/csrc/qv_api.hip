@@ -53,7 +53,7 @@ int check_params(const qv_params *p) {
   if (p->deriv_order != 2 && p->deriv_order != 4) return QV_ERR_ARG;
   if (p->precision != QV_FP64 && p->precision != QV_FP32C) return QV_ERR_ARG;
   if (!(p->quant_circ > 0.) || !(p->corea > 0.)) return QV_ERR_ARG;
-  if (p->reserved != 0) return QV_ERR_ARG;
+  if (p->flags & ~QV_FLAG_REMOVE_MEAN_POS) return QV_ERR_ARG;
   return QV_OK;
 }
 
@@ -292,6 +292,54 @@ int host_derivative(qv_ctx *ctx, const double *in, double *out, bool vjp) {
   return QV_OK;
 }
 
+// the boundary step of run.f90:82-90 on the device state.  Periodic: the
+// wrap.  openx / openxy: a device check for points beyond an open face
+// first; without any, the wrap of the periodic axes; with some, the state's
+// positions and links go to the host, boundary_step_host runs in slot order
+// and they come back, and the cleared slots' other arrays are zeroed.
+int boundary_step(Ctx &c) {
+  if (c.p.boundary != QV_BC_OPENX && c.p.boundary != QV_BC_OPENXY) {
+    TRY(launch_enforce_periodic(c));
+    return QV_OK;
+  }
+  int32_t *flag = c.counters + 6;
+  TRY(hipMemsetAsync(flag, 0, sizeof(int32_t), c.stream));
+  TRY(launch_open_check(c, flag));
+  TRY(hipMemcpyAsync(c.h_counters + 6, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  TRY(hipStreamSynchronize(c.stream));
+  if (c.h_counters[6] == 0) {
+    TRY(launch_enforce_periodic(c));
+    return QV_OK;
+  }
+  const size_t n = (size_t)c.pcount;
+  std::vector<double> hx(3 * n);
+  std::vector<int32_t> hinf(n), hbeh(n), cleared(n);
+  int rc = download_vec(c, c.x, hx.data());
+  if (rc) return rc;
+  TRY(hipMemcpyAsync(hinf.data(), c.infront, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                     c.stream));
+  TRY(hipMemcpyAsync(hbeh.data(), c.behind, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                     c.stream));
+  TRY(hipStreamSynchronize(c.stream));
+  int32_t ncleared = 0;
+  boundary_step_host(c.p, c.pcount, hx.data(), hinf.data(), hbeh.data(), c.boundary_count,
+                     c.boundary_length, cleared.data(), ncleared);
+  rc = upload_vec(c, hx.data(), c.x);
+  if (rc) return rc;
+  TRY(hipMemcpyAsync(c.infront, hinf.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c.stream));
+  TRY(hipMemcpyAsync(c.behind, hbeh.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c.stream));
+  // the build's slot buffer (cap >= pcount ints) carries the list; no tree
+  // build runs between here and the next velocity stage
+  TRY(hipMemcpyAsync(c.slot, cleared.data(), (size_t)ncleared * sizeof(int32_t),
+                     hipMemcpyHostToDevice, c.stream));
+  TRY(launch_clear_slots(c, ncleared, c.slot));
+  TRY(hipStreamSynchronize(c.stream));
+  c.nact -= ncleared;
+  return QV_OK;
+}
+
 int set_positions_device(Ctx &c, const double *d_x) {
   if (!d_x) return QV_OK;
   TRY(launch_aos_to_soa(c, d_x, c.x));
@@ -472,8 +520,17 @@ int qv_enforce_periodic(qv_ctx *ctx) {
   if (!ctx) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (c->pcount == 0) return QV_ERR_STATE;
-  TRY(launch_enforce_periodic(*c));
+  const int rc = boundary_step(*c);
+  if (rc) return rc;
   TRY(hipStreamSynchronize(c->stream));
+  return QV_OK;
+}
+
+int qv_boundary_removed(qv_ctx *ctx, int64_t *count, double *length) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (count) *count = c->boundary_count;
+  if (length) *length = c->boundary_length;
   return QV_OK;
 }
 
@@ -484,7 +541,7 @@ int qv_step(qv_ctx *ctx, int32_t nsteps) {
     int rc = velocity_stage(*c);
     if (rc) return rc;
     TRY(launch_ab_update(*c));
-    TRY(launch_enforce_periodic(*c));
+    if ((rc = boundary_step(*c))) return rc;
     c->have_velocity = false;
   }
   TRY(hipStreamSynchronize(c->stream));
@@ -503,6 +560,21 @@ int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, doubl
     if (rc) return rc;
     TRY(hipStreamSynchronize(c->stream));
   }
+  return QV_OK;
+}
+
+int qv_download_links(qv_ctx *ctx, int32_t *infront, int32_t *behind) {
+  if (!ctx) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  const size_t n = (size_t)c->pcount;
+  if (infront)
+    TRY(hipMemcpyAsync(infront, c->infront, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       c->stream));
+  if (behind)
+    TRY(hipMemcpyAsync(behind, c->behind, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       c->stream));
+  TRY(hipStreamSynchronize(c->stream));
   return QV_OK;
 }
 

@@ -518,8 +518,12 @@ int qv_batch_create(const qv_params *params, int32_t nsys, const int32_t *counts
   if (p.velocity != QV_VEL_BS || !(p.box_size > 0.) || !(p.corea > 0.) ||
       (p.deriv_order != 2 && p.deriv_order != 4) ||
       (p.precision != QV_FP64 && p.precision != QV_FP32C) || p.boundary < QV_BC_OPEN ||
-      p.boundary > QV_BC_OPENXY || p.reserved != 0)
+      p.boundary > QV_BC_OPENXY || p.flags != 0)
     return QV_ERR_ARG;
+  // the persistent launch wraps periodic axes but cannot remove a loop that
+  // leaves through an open face (boundary_loop_remove): open and periodic
+  // ensembles only
+  if (p.boundary == QV_BC_OPENX || p.boundary == QV_BC_OPENXY) return QV_ERR_ARG;
   int64_t tot = 0;
   for (int s = 0; s < nsys; ++s) {
     if (counts[s] < 3) return QV_ERR_ARG;

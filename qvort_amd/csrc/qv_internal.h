@@ -133,6 +133,9 @@ struct Ctx {
   Vec3 aw, ar;
   int32_t adj_cap = 0;
   bool links_consistent = false;  // infront(behind(i)) == i for every point
+  // boundary_loop_remove_count / _length of the boundary steps so far
+  int64_t boundary_count = 0;
+  double boundary_length = 0.;
   hipEvent_t ev[6] = {};
   // a second stream (default priority) for the walk order's sort, which
   // overlaps the rest of the tree build: aux_ev[0] = build data ready,
@@ -161,6 +164,21 @@ struct AccF64 {
   __device__ inline void get(double &a, double &b, double &c) const { a = s0; b = s1; c = s2; }
 };
 
+// the per-slot arrays clear_particle zeroes besides x and the links
+struct ClearArgs {
+  static constexpr int kArrays = 27;  // u, u1, u2, u_LI, u_BS, 4 ghosts; 3 axes each
+  double *a[kArrays];
+};
+
+// the boundary step of openx / openxy boxes on host arrays (x(3,pcount),
+// 1-based links), in the reference's slot order: enforce_periodic_yz / _z
+// (periodic.f90:217-276) with boundary_loop_remove (periodic.f90:306-327).
+// Cleared slots get x = 0 and links 0 and are appended to `cleared`;
+// count / length accumulate boundary_loop_remove_count / _length.
+void boundary_step_host(const qv_params &p, int32_t pcount, double *x, int32_t *inf,
+                        int32_t *beh, int64_t &count, double &length, int32_t *cleared,
+                        int32_t &ncleared);
+
 // launchers (each returns hipError_t of the launch)
 hipError_t launch_aos_to_soa(Ctx &c, const double *d_aos, Vec3 dst);
 hipError_t launch_soa_to_aos(Ctx &c, Vec3 src, double *d_aos);
@@ -168,6 +186,10 @@ hipError_t launch_ghost_lia(Ctx &c);
 hipError_t launch_ab_update(Ctx &c);
 hipError_t launch_rk3_update(Ctx &c, int stage);
 hipError_t launch_enforce_periodic(Ctx &c);
+// flag |= 1 when an active point lies beyond an open face (openx, openxy)
+hipError_t launch_open_check(Ctx &c, int32_t *flag);
+// clear_particle's zeroes in the m slots of d_list (device)
+hipError_t launch_clear_slots(Ctx &c, int m, const int32_t *d_list);
 hipError_t launch_direct(Ctx &c);
 // 21-bit Hilbert-curve keys of n points (x[i*stride], y[..], z[..]) on a
 // 128^3 grid of the box, idx[i] = i; with inf, points with inf[i] = 0 get

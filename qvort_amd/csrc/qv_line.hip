@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "qv_internal.h"
 #include "qv_lia.h"
 #include "qvort_gpu.h"
 
@@ -233,6 +234,66 @@ struct Line {
     return QV_OK;
   }
 };
+
+}  // namespace
+
+// The boundary step of an openx / openxy box on host arrays, slot by slot
+// as enforce_periodic_yz / _z run (periodic.f90:217-276): a point's periodic
+// axes are wrapped before its open axes are tested (openx: y, z, then x;
+// openxy: x, then y, then z is wrapped), and a point beyond an open face
+// removes its whole loop at once (boundary_loop_remove, periodic.f90:306-327).
+// The removal's length counter adds the segments of the loop with the
+// positions of the moment -- its points before the slot already wrapped,
+// those after it not yet -- so the order is part of the result.
+void boundary_step_host(const qv_params &p, int32_t pcount, double *x, int32_t *inf,
+                        int32_t *beh, int64_t &count, double &length, int32_t *cleared,
+                        int32_t &ncleared) {
+  const double L = p.box_size;
+  const bool openx = p.boundary == QV_BC_OPENX;
+  auto wrap = [L](double &v) {
+    if (v > (L / 2.)) v = v - L;
+    else if (v < (-L / 2.)) v = v + L;
+  };
+  auto outside = [L](double v) { return v > (L / 2.) || v < (-L / 2.); };
+  // the loop through `start`, cleared slot by slot (clear_particle's x and
+  // links; the device zeroes the rest of the cleared slots)
+  auto remove_loop = [&](int start) {
+    count += 1;
+    int cur = start;
+    for (int32_t it = 0; it < pcount; ++it) {
+      const int nxt = inf[cur] - 1;
+      if (nxt < 0) break;  // not a closed loop; the reference would read f(0)
+      if (nxt != start) {
+        const double *a = x + 3 * (size_t)cur, *b = x + 3 * (size_t)nxt;
+        length = length + std::sqrt(((a[0] - b[0]) * (a[0] - b[0]) +
+                                     (a[1] - b[1]) * (a[1] - b[1])) +
+                                    (a[2] - b[2]) * (a[2] - b[2]));
+      }
+      double *c = x + 3 * (size_t)cur;
+      c[0] = c[1] = c[2] = 0.;
+      inf[cur] = beh[cur] = 0;
+      cleared[ncleared++] = cur;
+      cur = nxt;
+      if (cur == start) break;
+    }
+  };
+  for (int i = 0; i < pcount; ++i) {
+    if (inf[i] == 0) continue;
+    double *xi = x + 3 * (size_t)i;
+    if (openx) {
+      wrap(xi[1]);
+      wrap(xi[2]);
+      if (outside(xi[0])) remove_loop(i);
+    } else {
+      // a cleared point has x = 0 and passes the later tests
+      if (outside(xi[0])) remove_loop(i);
+      if (outside(xi[1])) remove_loop(i);
+      wrap(xi[2]);
+    }
+  }
+}
+
+namespace {
 
 // the arrays the operations index must be present and the links must stay
 // inside the slots in use

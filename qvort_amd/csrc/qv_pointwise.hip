@@ -169,8 +169,9 @@ k_rk3_update(int n, double dt, int stage, const int32_t *__restrict__ inf,
   }
 }
 
-// enforce_periodic / _yz / _z wrap part (periodic.f90:188-276); loop removal
-// of the open axes is host topology and not done here
+// enforce_periodic / _yz / _z wrap part (periodic.f90:188-276), for states
+// with no point beyond an open face (k_open_check); loops that leave through
+// one are removed on the host (boundary_step_host)
 __global__ void k_wrap(int n, double L, int k0, const int32_t *__restrict__ inf,
                        double *__restrict__ X, double *__restrict__ Y, double *__restrict__ Z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,6 +181,93 @@ __global__ void k_wrap(int n, double L, int k0, const int32_t *__restrict__ inf,
     const double v = A[k][i];
     if (v > (L / 2.)) A[k][i] = v - L;
     else if (v < (-L / 2.)) A[k][i] = v + L;
+  }
+}
+
+// an active point beyond an open face of an openx (x) or openxy (x, y) box:
+// the loops to remove (enforce_periodic_yz / _z, periodic.f90:217-276)
+__global__ void k_open_check(int n, double L, int nopen, const int32_t *__restrict__ inf,
+                             const double *__restrict__ X, const double *__restrict__ Y,
+                             int32_t *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool out = false;
+  if (i < n && inf[i] != 0) {
+    const double a = X[i];
+    out = a > (L / 2.) || a < (-L / 2.);
+    if (nopen > 1) {
+      const double b = Y[i];
+      out = out || b > (L / 2.) || b < (-L / 2.);
+    }
+  }
+  if (__ballot(out) && (threadIdx.x % kWave) == 0) atomicOr(flag, 1);
+}
+
+// zero what clear_particle zeroes (general.f90:11-22) in the listed slots
+// whose positions and links came back from the host: velocity, AB history
+// u1 and u2 (not u3), the LIA and nonlocal parts, the ghosts
+__global__ void k_clear_slots(int m, const int32_t *__restrict__ list, ClearArgs A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const int i = list[q];
+  for (int k = 0; k < ClearArgs::kArrays; ++k) A.a[k][i] = 0.;
+}
+
+// remove_mean_pos (timestep.f90:86-92), first pass: per block the sums of
+// x and y over the active slots of its range (fixed order: a strided sum
+// per thread, then a tree over the block)
+constexpr int kMeanBlocks = 512;
+
+__global__ void __launch_bounds__(kBlock) k_mean_partial(int n, const int32_t *__restrict__ inf,
+                                                         const double *__restrict__ X,
+                                                         const double *__restrict__ Y,
+                                                         double *__restrict__ part) {
+  __shared__ double sx[kBlock], sy[kBlock];
+  const int per = (n + kMeanBlocks - 1) / kMeanBlocks;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  double a = 0., b = 0.;
+  for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock)
+    if (inf[i] > 0) {
+      a = a + X[i];
+      b = b + Y[i];
+    }
+  sx[threadIdx.x] = a;
+  sy[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sx[threadIdx.x] = sx[threadIdx.x] + sx[threadIdx.x + h];
+      sy[threadIdx.x] = sy[threadIdx.x] + sy[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sx[0];
+    part[2 * blockIdx.x + 1] = sy[0];
+  }
+}
+
+// second pass: every block adds the partial sums in block order (the same
+// sums everywhere), mean = sum / count, subtracted from every slot (empty
+// ones too, as f(:)%x(1) - mean_x does)
+__global__ void __launch_bounds__(kBlock) k_mean_apply(int n, int nact,
+                                                       const double *__restrict__ part,
+                                                       double *__restrict__ X,
+                                                       double *__restrict__ Y) {
+  __shared__ double m[2];
+  if (threadIdx.x == 0) {
+    double a = 0., b = 0.;
+    for (int k = 0; k < kMeanBlocks; ++k) {
+      a = a + part[2 * k];
+      b = b + part[2 * k + 1];
+    }
+    m[0] = a / (double)nact;
+    m[1] = b / (double)nact;
+  }
+  __syncthreads();
+  const double mx = m[0], my = m[1];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    X[i] = X[i] - mx;
+    Y[i] = Y[i] - my;
   }
 }
 
@@ -248,6 +336,37 @@ hipError_t launch_ab_update(Ctx &c) {
   hipLaunchKernelGGL(k_ab_update, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
                      c.p.dt, c.infront, c.x.x, c.x.y, c.x.z, c.u.x, c.u.y, c.u.z, c.u1.x, c.u1.y,
                      c.u1.z, c.u2.x, c.u2.y, c.u2.z, c.u3.x, c.u3.y, c.u3.z);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !(c.p.flags & QV_FLAG_REMOVE_MEAN_POS) || c.nact == 0) return e;
+  // the walk's part buffer (3 cap >= 2 kMeanBlocks doubles) holds the
+  // partial sums; the walk is done with it once a stage has finished
+  hipLaunchKernelGGL(k_mean_partial, dim3(kMeanBlocks), dim3(kBlock), 0, c.stream, c.pcount,
+                     c.infront, c.x.x, c.x.y, c.walk_part);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mean_apply, dim3(grid(c.pcount) < 1024 ? grid(c.pcount) : 1024),
+                     dim3(kBlock), 0, c.stream, c.pcount, c.nact, c.walk_part, c.x.x, c.x.y);
+  return hipGetLastError();
+}
+
+hipError_t launch_open_check(Ctx &c, int32_t *flag) {
+  const int nopen = c.p.boundary == QV_BC_OPENX ? 1 : (c.p.boundary == QV_BC_OPENXY ? 2 : 0);
+  if (nopen == 0 || c.pcount == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_open_check, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
+                     c.p.box_size, nopen, c.infront, c.x.x, c.x.y, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_slots(Ctx &c, int m, const int32_t *d_list) {
+  if (m <= 0) return hipSuccess;
+  ClearArgs A;
+  Vec3 vs[] = {c.u, c.u1, c.u2, c.uli, c.ubs, c.gi, c.gb, c.gii, c.gbb};
+  int k = 0;
+  for (const Vec3 &v : vs) {
+    A.a[k++] = v.x;
+    A.a[k++] = v.y;
+    A.a[k++] = v.z;
+  }
+  hipLaunchKernelGGL(k_clear_slots, dim3(grid(m)), dim3(kBlock), 0, c.stream, m, d_list, A);
   return hipGetLastError();
 }
 

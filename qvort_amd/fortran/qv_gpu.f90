@@ -6,7 +6,9 @@
 !>   ghostp (periodic.f90:8-17) + construct_tree (tree.f90:44-80) + the
 !>   velocity loop of pmotion (timestep.f90:41-47),
 !> qv_ab_step the Adams-Bashforth ladder of pmotion (timestep.f90:48-71),
-!> qv_enforce_periodic the wrap of enforce_periodic (periodic.f90:188-213),
+!> qv_enforce_periodic the boundary step of run.f90:82-90 (enforce_periodic,
+!> or enforce_periodic_yz / _z with their loop removal at open faces,
+!> periodic.f90:188-327),
 !> qv_closest the neighbour search pclose_tree / pclose (tree.f90:219-278,
 !> line.f90:205-234) that precon reads, qv_line_pinsert / qv_line_precon /
 !> qv_line_premove the host line operations of run.f90:60-72 (pinsert,
@@ -26,6 +28,7 @@ module qv_gpu
   integer(c_int32_t), parameter, public :: QV_FP64 = 0, QV_FP32C = 1
   integer(c_int32_t), parameter, public :: QV_PROBE_GHOSTP = 1, QV_PROBE_REFERENCE_SCALE = 2, &
                                            QV_PROBE_EMPTY_CORE = 4
+  integer(c_int32_t), parameter, public :: QV_FLAG_REMOVE_MEAN_POS = 1
   integer(c_int), parameter, public :: QV_OK = 0, QV_ERR_ARG = 1, QV_ERR_HIP = 2, &
                                        QV_ERR_SINGULAR = 3, QV_ERR_TREE_DEPTH = 4, &
                                        QV_ERR_STATE = 5, QV_ERR_NO_DEVICE = 6, &
@@ -36,7 +39,7 @@ module qv_gpu
   type, bind(C), public :: qv_params
     real(c_double) :: quant_circ, corea, box_size, delta, tree_theta, dt
     integer(c_int32_t) :: velocity, boundary, tree_extra_correction, deriv_order
-    integer(c_int32_t) :: precision, reserved
+    integer(c_int32_t) :: precision, flags
   end type
 
   type, bind(C), public :: qv_timings
@@ -62,6 +65,7 @@ module qv_gpu
 
   public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
             qv_velocity, qv_ab_step, qv_rk3_step, qv_enforce_periodic, qv_step, qv_download, &
+            qv_download_links, qv_boundary_removed, &
             qv_tree_dump, qv_closest, qv_probe_velocity, qv_vjp, qv_jvp, qv_get_timings, &
             qv_download_ghosts, qv_line_pinsert, qv_line_precon, qv_line_premove, &
             qv_host_alloc, qv_host_free, &
@@ -117,6 +121,22 @@ module qv_gpu
     integer(c_int) function qv_download(ctx, x, u, u1, u2, u3) bind(C, name='qv_download')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, x, u, u1, u2, u3
+    end function
+    !> the device links; they change on the device only when the boundary
+    !> step removes loops at an open face
+    integer(c_int) function qv_download_links(ctx, infront, behind) &
+        bind(C, name='qv_download_links')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: infront(*), behind(*)
+    end function
+    !> boundary_loop_remove_count / _length of the context's boundary steps
+    integer(c_int) function qv_boundary_removed(ctx, count, length) &
+        bind(C, name='qv_boundary_removed')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: count
+      real(c_double), intent(out) :: length
     end function
     integer(c_int) function qv_tree_dump(ctx, out, max_nodes, nnodes) bind(C, name='qv_tree_dump')
       import :: c_int, c_ptr, c_int64_t

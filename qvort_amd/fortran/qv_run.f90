@@ -32,8 +32,14 @@
 !> run.in labels used: nsteps shots dt delta box_size velocity boundary
 !> quant_circ corea tree_theta tree_extra_correction deriv_order runge_kutta initf pcount
 !> tree_print restart_rewind recon_shots mesh_size mesh_shots mesh_print_delay
-!> normal_velocity, and gpu_precision (fp64 | fp32c), line_ops (T | F) and
-!> mesh_reference_scale (T | F), this driver's own.
+!> normal_velocity remove_mean_pos switch_off_recon, and gpu_precision
+!> (fp64 | fp32c), line_ops (T | F) and mesh_reference_scale (T | F), this
+!> driver's own.  Labels that change the motion and are not carried here
+!> (recon_type other than original, delta_adapt, phonon_emission,
+!> hyperviscosity, alpha, force, sticky_z_boundary, kelvin_wave_casc,
+!> svistunov_hamiltonian, killing_sphere_on, moving_box,
+!> multiple_initial_loop, particles, vortex injection) stop the run when
+!> set to anything but their defaults.
 !>
 !> var.dat holds f(:) as the reference's type qvort (cdata.f90:20-38); the
 !> components this driver does not carry (x_interp, acc, u_mf, u_s_LI,
@@ -76,6 +82,8 @@ program qv_run
   integer :: mesh_size = 0, mesh_shots = 100, mesh_print_delay = 0
   logical :: mesh_reference_scale = .false.
   character(len=30) :: normal_velocity = 'zero'
+  logical :: remove_mean_pos = .false., switch_off_recon = .false.
+  integer :: inject_skip = 10000000
   type(qv_line_counts) :: counts
   ! the mesh points (3, mesh_size**3) in the order of mesh(k,j,i) and their
   ! superfluid velocity
@@ -145,7 +153,7 @@ program qv_run
   p%tree_extra_correction = merge(1, 0, tree_extra_correction)
   p%deriv_order = merge(4, 2, deriv_order == 'fourth')
   p%precision = merge(QV_FP32C, QV_FP64, gpu_precision == 'fp32c')
-  p%reserved = 0
+  p%flags = merge(QV_FLAG_REMOVE_MEAN_POS, 0, remove_mean_pos)
 
   call qv_check(qv_create(p, ctx), 'qv_create')
   call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
@@ -176,6 +184,10 @@ program qv_run
     end if
     if (line_ops) call line_step()
     call qv_check(qv_enforce_periodic(ctx), 'qv_enforce_periodic')
+    ! in openx / openxy boxes the boundary step removes loops that leave
+    ! through an open face: the host's links follow
+    if (p%boundary == QV_BC_OPENX .or. p%boundary == QV_BC_OPENXY) &
+      call qv_check(qv_download_links(ctx, infront, behind), 'qv_download_links')
     t = t + dt
     if (mod(itime, shots) == 0) then
       ! after the line operations the host holds the step's velocities (an
@@ -273,10 +285,20 @@ contains
           read(buffer, *, iostat=ios) mesh_reference_scale
         case ('normal_velocity')
           read(buffer, *, iostat=ios) normal_velocity
+        case ('remove_mean_pos')
+          read(buffer, *, iostat=ios) remove_mean_pos
+        case ('switch_off_recon')
+          read(buffer, *, iostat=ios) switch_off_recon
+        case ('inject_skip')
+          read(buffer, *, iostat=ios) inject_skip
+        case default
+          call refuse_unsupported(label, buffer)
       end select
       ios = 0
     end do
     close(fh)
+    ! vortex_inject runs when inject_skip divides itime (run.f90:38-40)
+    if (inject_skip <= nsteps) call unsupported('inject_skip', 'vortex injection')
     if (.not. box_size > 0._c_double) call stop_with('init_setup:', 'box size is less than zero')
     ! init_setup's checks of the velocity choice (initial.f90:355-363)
     if (velocity == 'Tree') then
@@ -285,6 +307,45 @@ contains
       if (tree_theta > 0.6_c_double) call warning('init_setup', &
         'tree_theta>0.6, the vortices may be unstable')
     end if
+  end subroutine
+
+  !> run.in switches of the reference that change the motion of the points
+  !> and that this driver does not carry: a run.in that sets one of them to
+  !> anything but its default (cdata.f90:196-416) stops here rather than run
+  !> different physics.  Labels of output, diagnostics and initial
+  !> conditions the driver does not read are ignored, as before.
+  subroutine refuse_unsupported(label, value)
+    character(len=*), intent(in) :: label, value
+    character(len=30) :: word
+    logical :: flag
+    real(c_double) :: a(2)
+    integer :: n, ios
+    select case (label)
+      case ('recon_type')
+        read(value, *, iostat=ios) word
+        if (ios == 0 .and. word /= 'original') call unsupported(label, word)
+      case ('force')
+        read(value, *, iostat=ios) word
+        if (ios == 0 .and. word /= 'off') call unsupported(label, word)
+      case ('delta_adapt', 'phonon_emission', 'hyperviscosity', 'sticky_z_boundary', &
+            'kelvin_wave_casc', 'svistunov_hamiltonian', 'killing_sphere_on', 'moving_box', &
+            'multiple_initial_loop')
+        read(value, *, iostat=ios) flag
+        if (ios == 0 .and. flag) call unsupported(label, 'T')
+      case ('alpha')
+        a = 0._c_double
+        read(value, *, iostat=ios) a
+        if (any(a /= 0._c_double)) call unsupported(label, 'mutual friction')
+      case ('quasi_pcount', 'part_count')
+        read(value, *, iostat=ios) n
+        if (ios == 0 .and. n > 0) call unsupported(label, 'particles')
+    end select
+  end subroutine
+
+  subroutine unsupported(label, value)
+    character(len=*), intent(in) :: label, value
+    call stop_with('qv_run:read_run_in', trim(label)//' '//trim(value)// &
+                   ' changes the motion and is not implemented by this driver')
   end subroutine
 
   subroutine init_filament()
@@ -688,7 +749,7 @@ contains
       call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
       call qv_check(qv_closest(ctx, closest, closestd), 'qv_closest')
       ls = line_state()
-      call qv_check(qv_line_precon(p, ls, counts), 'line.mod:precon')
+      if (.not. switch_off_recon) call qv_check(qv_line_precon(p, ls, counts), 'line.mod:precon')
       call qv_check(qv_line_premove(p, ls, counts), 'line.mod:premove')
     end if
     call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')

@@ -174,6 +174,22 @@ def ring_ensemble(n_systems: int, *, nmin: int = 200, nmax: int = 5000, rmin: fl
     return out
 
 
+def combine(*fils: Filaments) -> Filaments:
+    """The filaments of several records in one f(:), in the order given
+    (links renumbered; empty slots stay empty)."""
+    xs, infs, behs, starts, lens = [], [], [], [], []
+    off = 0
+    for f in fils:
+        xs.append(f.x)
+        infs.append(np.where(f.infront != 0, f.infront + off, 0).astype(np.int32))
+        behs.append(np.where(f.behind != 0, f.behind + off, 0).astype(np.int32))
+        starts.append(f.loop_start + off)
+        lens.append(f.loop_len)
+        off += f.pcount
+    return Filaments(np.concatenate(xs), np.concatenate(infs), np.concatenate(behs),
+                     np.concatenate(starts), np.concatenate(lens))
+
+
 def with_holes(fil: Filaments, n_holes: int, seed: int = 7) -> Filaments:
     """Insert empty slots (infront=behind=0) at seeded positions, renumbering
     the links, as Qvort's f(:) holds after point removals (line.f90:114-161)."""

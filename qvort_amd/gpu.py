@@ -29,9 +29,11 @@ PREC = {"fp64": 0, "fp32c": 1}
 QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE, \
     QV_ERR_NO_DEVICE, QV_ERR_LINE_EMPTY, QV_ERR_CAPACITY, QV_ERR_SCRATCH = range(10)
 QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE, QV_PROBE_EMPTY_CORE = 1, 2, 4
+QV_FLAG_REMOVE_MEAN_POS = 1
 
 EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
-           "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic", "qv_step", "qv_download",
+           "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic",
+           "qv_boundary_removed", "qv_step", "qv_download", "qv_download_links",
            "qv_tree_dump", "qv_closest", "qv_probe_velocity", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
            "qv_vjp_device", "qv_jvp_device", "qv_vjp_device_noalloc", "qv_get_timings",
            "qv_batch_create",
@@ -47,7 +49,7 @@ class QvParams(C.Structure):
                 ("delta", C.c_double), ("tree_theta", C.c_double), ("dt", C.c_double),
                 ("velocity", C.c_int32), ("boundary", C.c_int32),
                 ("tree_extra_correction", C.c_int32), ("deriv_order", C.c_int32),
-                ("precision", C.c_int32), ("reserved", C.c_int32)]
+                ("precision", C.c_int32), ("flags", C.c_int32)]
 
 
 class QvTimings(C.Structure):
@@ -80,14 +82,14 @@ class QvortError(RuntimeError):
 
 def params(*, quant_circ=9.97e-4, corea=8.244023e-9, box_size=0.1, delta=5e-4, tree_theta=0.0,
            dt=1e-6, velocity="Tree", boundary="periodic", tree_extra_correction=True,
-           deriv_order=2, precision="fp64") -> QvParams:
+           deriv_order=2, precision="fp64", remove_mean_pos=False) -> QvParams:
     """run.in parameters of the hot path; defaults are the reference's
     (cdata.f90:196-416) where it has one."""
     if isinstance(deriv_order, str):
         deriv_order = {"second": 2, "fourth": 4}[deriv_order]
     return QvParams(quant_circ, corea, box_size, delta, tree_theta, dt, VEL[velocity],
                     BC[boundary], 1 if tree_extra_correction else 0, int(deriv_order),
-                    PREC[precision], 0)
+                    PREC[precision], QV_FLAG_REMOVE_MEAN_POS if remove_mean_pos else 0)
 
 
 _lib = None
@@ -126,8 +128,10 @@ def lib():
         L.qv_ab_step.argtypes = [vp]
         L.qv_rk3_step.argtypes = [vp]
         L.qv_enforce_periodic.argtypes = [vp]
+        L.qv_boundary_removed.argtypes = [vp, i64, d]
         L.qv_step.argtypes = [vp, C.c_int32]
         L.qv_download.argtypes = [vp, d, d, d, d, d]
+        L.qv_download_links.argtypes = [vp, i32, i32]
         L.qv_tree_dump.argtypes = [vp, d, C.c_int64, i64]
         L.qv_get_timings.argtypes = [vp, P(QvTimings)]
         L.qv_closest.argtypes = [vp, i32, d]
@@ -250,6 +254,13 @@ class Context:
     def enforce_periodic(self) -> None:
         _check(lib().qv_enforce_periodic(self._h), "qv_enforce_periodic")
 
+    def boundary_removed(self) -> tuple[int, float]:
+        """(boundary_loop_remove_count, _length) of this context's boundary steps."""
+        n, length = C.c_int64(0), C.c_double(0.0)
+        _check(lib().qv_boundary_removed(self._h, C.byref(n), C.byref(length)),
+               "qv_boundary_removed")
+        return int(n.value), float(length.value)
+
     def step(self, nsteps: int) -> None:
         _check(lib().qv_step(self._h, nsteps), "qv_step")
 
@@ -267,6 +278,21 @@ class Context:
         arrs = self._outs(5, out)
         _check(lib().qv_download(self._h, *[_dp(a) for a in arrs]), "qv_download")
         return tuple(arrs)
+
+    def download_links(self, infront=None, behind=None):
+        """(infront, behind) of the device state; the given C-contiguous
+        int32 (pcount,) arrays are filled in place when passed."""
+        n = self.pcount
+        arrs = []
+        for a in (infront, behind):
+            if a is None:
+                a = np.zeros(n, dtype=np.int32)
+            elif a.dtype != np.int32 or not a.flags.c_contiguous or a.shape[0] < n:
+                raise ValueError("links must be C-contiguous int32 arrays of >= pcount entries")
+            arrs.append(a)
+        _check(lib().qv_download_links(self._h, *[a.ctypes.data_as(C.POINTER(C.c_int32))
+                                                  for a in arrs]), "qv_download_links")
+        return arrs[0][:n], arrs[1][:n]
 
     def download_ghosts(self, out=None):
         """(ghosti, ghostb, ghostii, ghostbb) of the last ghostp; out: four

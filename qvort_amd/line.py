@@ -123,14 +123,19 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
     device, pinsert runs on the host, and when `search` is set (recon_shots
     divides itime) the device searches the step's tree for every slot's
     closest point before precon and premove run on the host.  The result is
-    uploaded with its AB history; the caller then wraps (qv_enforce_periodic)
-    as the reference does after its ghostp.  The host owns the topology: `st`
-    holds the current links (and closest/closestd) between calls; the device
-    sends back positions, velocities and ghosts only."""
+    uploaded with its AB history; the caller then runs the boundary step
+    (qv_enforce_periodic) as the reference does after its ghostp.  The host
+    owns the topology: `st` holds the current links (and closest/closestd)
+    between calls; the device sends back positions, velocities and ghosts,
+    and in openx / openxy boxes the links, which its boundary step changes
+    when a loop leaves through an open face."""
     n = ctx.pcount
     if n > st.capacity:
         raise ValueError("the line state holds fewer slots than the device")
     st.pcount = n
+    if p.boundary in (gpu.BC["openx"], gpu.BC["openxy"]):
+        # the device's boundary step may have removed loops at an open face
+        ctx.download_links(st._infront, st._behind)
     # straight into the state's arrays (their first n rows are contiguous)
     ctx.download(out=[getattr(st, "_" + k)[:n] for k in ("x", "u", "u1", "u2", "u3")])
     ctx.download_ghosts(out=[getattr(st, "_" + k)[:n]

@@ -103,3 +103,8 @@ def rel_err(u, ref, mask=None):
 def oracle_lib():
     from oracle import port
     return port.lib()
+
+
+def bnd_names():
+    with open(os.path.join(GOLDEN, "bnd_index.json")) as fh:
+        return sorted(json.load(fh))

@@ -12,6 +12,7 @@ case records the run.in parameters it used.  Run from the repository root:
     python tests/golden/make_golden.py --formats    # var.dat and tree_mesh files
     python tests/golden/make_golden.py --line       # pinsert / precon / premove
     python tests/golden/make_golden.py --mesh       # mesh velocity files
+    python tests/golden/make_golden.py --boundary   # open-face loop removal, remove_mean_pos
 
 The neighbour-search cases (closest_*.npz, index in closest_index.json) run
 one step of ghostp, construct_tree, pmotion, optionally pinsert, then
@@ -41,6 +42,13 @@ The mesh cases (mesh_*.npz, index in mesh_index.json) run nsteps of the main
 loop with mesh_size > 0: the reference's mesh_velocity after the motion and
 data/mesh%03d.dat as print_mesh writes it.  They store the input state and
 the bytes of every mesh file (mesh_<n>).
+
+The boundary cases (bnd_*.npz, index in bnd_index.json) run nsteps of the
+main loop on states with loops across an open face of openx / openxy boxes
+(enforce_periodic_yz / _z remove them, boundary_loop_remove) or with
+remove_mean_pos T, and store the input, the first step's velocity, the
+positions, AB history and links after nsteps, and boundary_loop_remove's
+count and length.
 """
 from __future__ import annotations
 
@@ -316,7 +324,71 @@ def main_mesh():
         json.dump(index, fh, indent=1, sort_keys=True)
 
 
+def _ring(n, radius, centre, plane="xy"):
+    """A ring of n points in the given coordinate plane."""
+    r = G.single_ring(n, radius)
+    if plane == "xz":
+        r.x = r.x[:, [0, 2, 1]].copy()
+    r.x = r.x + np.asarray(centre, dtype=np.float64)
+    return r
+
+
+def boundary_cases():
+    L = 0.1
+    inner = G.random_loops(L, 5e-4, n_loops=3, seed=61)
+    # openx: one ring straddles x = +L/2 and one x = -L/2 (both removed after
+    # the first motion), one straddles y = +L/2 (wrapped); holes in f(:)
+    openx = G.with_holes(G.combine(inner, _ring(60, 0.004, (L / 2 - 0.0015, 0.01, 0.0)),
+                                   _ring(48, 0.003, (-L / 2 + 0.001, -0.02, 0.01), "xz"),
+                                   _ring(52, 0.0035, (0.02, L / 2 - 0.001, 0.0))), 12, seed=62)
+    # openxy: a ring straddles y = -L/2 (removed), one z = +L/2 (wrapped), one
+    # x = +L/2 (removed)
+    openxy = G.combine(_ring(56, 0.004, (0.0, -L / 2 + 0.0012, 0.0)),
+                       _ring(50, 0.0035, (0.01, 0.02, L / 2 - 0.001), "xz"),
+                       _ring(44, 0.003, (L / 2 - 0.0005, 0.0, -0.02)),
+                       G.random_loops(L, 5e-4, n_loops=2, seed=63))
+    kel = G.kelvin_line(256, L, amp_sigma=2e-3, seed=64)
+    base = dict(box_size=L, delta=5e-4, dt=1e-6, corea=1.3e-8, quant_circ=9.97e-4)
+    return [
+        ("bnd_openface_bs_openx", openx, dict(base, velocity="BS", tree_theta=0.0,
+                                              boundary="openx"), 3),
+        ("bnd_openface_tree_openxy", openxy, dict(base, velocity="Tree", tree_theta=0.4,
+                                                  boundary="openxy"), 3),
+        ("bnd_openface_tree_openx", openx, dict(base, velocity="Tree", tree_theta=0.5,
+                                                boundary="openx"), 2),
+        ("bnd_kelvin_meanpos_bs_periodic", kel, dict(base, velocity="BS", tree_theta=0.0,
+                                                     boundary="periodic",
+                                                     remove_mean_pos=True), 6),
+        ("bnd_kelvin_meanpos_tree_periodic", kel, dict(base, velocity="Tree", tree_theta=0.4,
+                                                       boundary="periodic",
+                                                       remove_mean_pos=True), 5),
+    ]
+
+
+def main_boundary():
+    index = {}
+    for name, fil, params, nsteps in boundary_cases():
+        r = refrun.run(fil, nsteps=nsteps, threads=1, topology=True, **params)
+        arrays = dict(x0=fil.x, infront=fil.infront, behind=fil.behind, u=r["u"],
+                      x_final=r["x"], u1_final=r["u1"], u2_final=r["u2"], u3_final=r["u3"],
+                      infront_final=r["infront_final"], behind_final=r["behind_final"],
+                      boundary=np.array([r["boundary_count"]], dtype=np.int64),
+                      boundary_length=np.array([r["boundary_length"]]))
+        meta = dict(params=params, nsteps=nsteps, pcount=fil.pcount, n_active=fil.n_active)
+        arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+        index[name] = meta
+        print(name, fil.pcount, "active", fil.n_active, "->",
+              int(np.count_nonzero(r["infront_final"])), "loops removed",
+              r["boundary_count"], "length", r["boundary_length"], flush=True)
+    with open(os.path.join(OUT, "bnd_index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+
+
 def main():
+    if "--boundary" in sys.argv[1:]:
+        main_boundary()
+        return
     if "--mesh" in sys.argv[1:]:
         main_mesh()
         return
