@@ -114,6 +114,19 @@ int qv_set_params(qv_ctx *ctx, const qv_params *params);
 /* copy f(:)%x, infront, behind to the device; resets the AB history to 0 */
 int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x,
               const int32_t *infront, const int32_t *behind);
+/* positions and links of f(:) after the host line operations of a step
+ * (pinsert, precon, premove), keeping the device's velocities and AB
+ * history: pcount may have grown (never shrunk); a slot that held no point
+ * and holds one now is a point pinsert made (u1 = u2 = 0, u the mean of
+ * the velocities of the point it follows and of that point's infront,
+ * line.f90:86-93, in pinsert's order along chains of new points); a slot
+ * that held a point and is empty now is cleared as clear_particle clears
+ * it (general.f90:11-22).  Call it right after pinsert (the new points'
+ * velocities are read off the links pinsert left) and after precon /
+ * premove.  The copies of a topology step are then x down, the ghosts down
+ * and x and the links up twice, instead of the whole state both ways. */
+int qv_upload_topology(qv_ctx *ctx, int32_t pcount, const double *x,
+                       const int32_t *infront, const int32_t *behind);
 /* AB history f(:)%u1..u3 (NULL = leave as is) */
 int qv_upload_history(qv_ctx *ctx, const double *u1, const double *u2,
                       const double *u3);

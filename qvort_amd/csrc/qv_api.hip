@@ -90,18 +90,42 @@ hipError_t vzero(Vec3 &v, size_t n, hipStream_t s) {
   } while (0)
 
 // slot buffers grow by a quarter at a time: pinsert's appended slots make
-// pcount creep up from one upload to the next
-int reserve(Ctx &c, int32_t n) {
+// pcount creep up from one upload to the next.  keep: the state arrays
+// (positions, links, velocities, AB history, ghosts) keep their first
+// pcount entries; the rest are scratch of a stage.
+int reserve(Ctx &c, int32_t n, bool keep = false) {
   if (n <= c.cap) return QV_OK;
   // the buffers below are freed: the walk order's sort may still use some
   if (c.aux_stream) TRY(hipStreamSynchronize(c.aux_stream));
+  TRY(hipStreamSynchronize(c.stream));
   const size_t cap = (size_t)n + (size_t)n / 4 + 1024;
   if (cap > (size_t)INT32_MAX) return QV_ERR_ARG;
+  const size_t old = keep ? (size_t)c.pcount : 0;
   Vec3 *vs[] = {&c.x, &c.gi, &c.gb, &c.gii, &c.gbb, &c.u, &c.uli, &c.ubs,
-                &c.u1, &c.u2, &c.u3, &c.xs, &c.gs};
-  for (Vec3 *v : vs) TRY(valloc(*v, cap));
-  TRY(dalloc(c.infront, cap));
-  TRY(dalloc(c.behind, cap));
+                &c.u1, &c.u2, &c.u3};
+  for (Vec3 *v : vs) {
+    Vec3 nv;
+    TRY(valloc(nv, cap));
+    if (old) {
+      TRY(hipMemcpy(nv.x, v->x, old * sizeof(double), hipMemcpyDeviceToDevice));
+      TRY(hipMemcpy(nv.y, v->y, old * sizeof(double), hipMemcpyDeviceToDevice));
+      TRY(hipMemcpy(nv.z, v->z, old * sizeof(double), hipMemcpyDeviceToDevice));
+    }
+    vfree(*v);
+    *v = nv;
+  }
+  int32_t *links[2] = {c.infront, c.behind};
+  for (int q = 0; q < 2; ++q) {
+    int32_t *nl = nullptr;
+    TRY(dalloc(nl, cap));
+    if (old) TRY(hipMemcpy(nl, links[q], old * sizeof(int32_t), hipMemcpyDeviceToDevice));
+    (void)hipFree(links[q]);
+    links[q] = nl;
+  }
+  c.infront = links[0];
+  c.behind = links[1];
+  TRY(valloc(c.xs, cap));
+  TRY(valloc(c.gs, cap));
   TRY(dalloc(c.stage, 3 * cap));
   TRY(dalloc(c.key0, cap));
   TRY(dalloc(c.key1, 2 * cap));
@@ -134,10 +158,13 @@ int upload_vec(Ctx &c, const double *host, Vec3 v) {
   return QV_OK;
 }
 
-int velocity_stage(Ctx &c) {
+// the launches of one velocity stage (ghostp + LIA, tree build, walk or
+// direct sum); the error flags (counters[1]) are not reset here, so that the
+// stages of one qv_step call accumulate them
+int velocity_stage_launch(Ctx &c) {
   if (c.pcount == 0) return QV_ERR_STATE;
   TRY(hipEventRecord(c.ev[0], c.stream));
-  TRY(hipMemsetAsync(c.counters, 0, 8 * sizeof(int32_t), c.stream));
+  TRY(hipMemsetAsync(c.counters + 2, 0, 6 * sizeof(int32_t), c.stream));
   TRY(hipMemsetAsync(c.evals, 0, 4 * sizeof(int64_t), c.stream));
   TRY(launch_ghost_lia(c));
   c.ghosts_valid = true;
@@ -159,6 +186,15 @@ int velocity_stage(Ctx &c) {
     TRY(launch_direct(c));
   }
   TRY(hipEventRecord(c.ev[3], c.stream));
+  c.tree_valid = c.p.tree_theta > 0.;
+  c.tree_pcount = c.pcount;
+  return QV_OK;
+}
+
+// after the last stage of a call: one copy of the counters and flags, the
+// timings of that stage, the singular-leaf error of any stage since the
+// flags were reset
+int velocity_stage_finish(Ctx &c) {
   TRY(hipMemcpyAsync(c.h_counters, c.counters, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
                      c.stream));
   TRY(hipMemcpyAsync(c.h_evals, c.evals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
@@ -191,9 +227,14 @@ int velocity_stage(Ctx &c) {
   }
   if (c.h_counters[1] & kFlagSingular) return QV_ERR_SINGULAR;
   c.have_velocity = true;
-  c.tree_valid = c.p.tree_theta > 0.;
-  c.tree_pcount = c.pcount;
   return QV_OK;
+}
+
+int velocity_stage(Ctx &c) {
+  TRY(hipMemsetAsync(c.counters + 1, 0, sizeof(int32_t), c.stream));
+  int rc = velocity_stage_launch(c);
+  if (rc) return rc;
+  return velocity_stage_finish(c);
 }
 
 struct DumpCtx {
@@ -478,6 +519,48 @@ int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infro
   return QV_OK;
 }
 
+int qv_upload_topology(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infront,
+                       const int32_t *behind) {
+  if (!ctx || !x || !infront || !behind) return QV_ERR_ARG;
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->pcount == 0) return QV_ERR_STATE;
+  if (pcount < c->pcount) return QV_ERR_ARG;  // slots are never given back
+  int32_t nact = 0;
+  bool consistent = true;
+  for (int32_t i = 0; i < pcount; ++i) {
+    const int32_t a = infront[i], b = behind[i];
+    if (a == 0) continue;
+    if (a < 1 || a > pcount || b < 1 || b > pcount) return QV_ERR_ARG;
+    if (infront[a - 1] == 0 || infront[b - 1] == 0) return QV_ERR_ARG;
+    if (behind[a - 1] != i + 1 || infront[b - 1] != i + 1) consistent = false;
+    ++nact;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
+  const int32_t n_old = c->pcount;
+  int rc = reserve(*c, pcount, true);
+  if (rc) return rc;
+  c->pcount = pcount;
+  // the new links go to the build's slot buffers (cap ints each), the
+  // marks to its node-count buffer; no tree build runs until the next stage
+  TRY(hipMemcpyAsync(c->slot, infront, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c->stream));
+  TRY(hipMemcpyAsync(c->slot_s, behind, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
+                     c->stream));
+  TRY(launch_topology_apply(*c, pcount, n_old, c->slot, c->slot_s, c->nnode_of));
+  TRY(hipMemcpyAsync(c->infront, c->slot, pcount * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                     c->stream));
+  TRY(hipMemcpyAsync(c->behind, c->slot_s, pcount * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                     c->stream));
+  rc = upload_vec(*c, x, c->x);
+  if (rc) return rc;
+  TRY(hipStreamSynchronize(c->stream));
+  c->nact = nact;
+  c->links_consistent = consistent;
+  c->have_velocity = false;
+  c->ghosts_valid = false;
+  return QV_OK;
+}
+
 int qv_upload_history(qv_ctx *ctx, const double *u1, const double *u2, const double *u3) {
   if (!ctx) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -537,15 +620,21 @@ int qv_boundary_removed(qv_ctx *ctx, int64_t *count, double *length) {
 int qv_step(qv_ctx *ctx, int32_t nsteps) {
   if (!ctx || nsteps < 0) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  // device-resident: the host waits once per stage only for the tree's node
+  // count (the node arrays are sized by it) and, in openx / openxy boxes,
+  // for the open-face check of the boundary step; the counters and the
+  // error flags are read once, after the last step
+  if (nsteps == 0) return QV_OK;
+  TRY(hipMemsetAsync(c->counters + 1, 0, sizeof(int32_t), c->stream));
   for (int32_t k = 0; k < nsteps; ++k) {
-    int rc = velocity_stage(*c);
+    int rc = velocity_stage_launch(*c);
     if (rc) return rc;
     TRY(launch_ab_update(*c));
     if ((rc = boundary_step(*c))) return rc;
-    c->have_velocity = false;
   }
-  TRY(hipStreamSynchronize(c->stream));
-  return QV_OK;
+  const int rc = velocity_stage_finish(*c);
+  c->have_velocity = false;
+  return rc;
 }
 
 int qv_download(qv_ctx *ctx, double *x, double *u, double *u1, double *u2, double *u3) {

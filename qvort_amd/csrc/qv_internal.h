@@ -188,6 +188,11 @@ hipError_t launch_rk3_update(Ctx &c, int stage);
 hipError_t launch_enforce_periodic(Ctx &c);
 // flag |= 1 when an active point lies beyond an open face (openx, openxy)
 hipError_t launch_open_check(Ctx &c, int32_t *flag);
+// the per-slot rules of a topology upload: new points (pinsert) and cleared
+// slots, from the device's old links and the uploaded new ones (device
+// buffers); mark: n_new ints of scratch
+hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *new_inf,
+                                 const int32_t *new_beh, int32_t *mark);
 // clear_particle's zeroes in the m slots of d_list (device)
 hipError_t launch_clear_slots(Ctx &c, int m, const int32_t *d_list);
 hipError_t launch_direct(Ctx &c);

@@ -212,6 +212,58 @@ __global__ void k_clear_slots(int m, const int32_t *__restrict__ list, ClearArgs
   for (int k = 0; k < ClearArgs::kArrays; ++k) A.a[k][i] = 0.;
 }
 
+// the topology upload after host line operations.  A slot that held no
+// point before (or lies past the old slot count) and holds one now is a
+// point pinsert made: u1 = u2 = 0 (line.f90:93; an appended slot's u3 = 0
+// too); a slot that held a point and is empty now was cleared by
+// clear_particle (general.f90:11-22): velocity, u1, u2, the LIA and
+// nonlocal parts and the ghosts are zeroed.  mark[s] = 1 for a new point.
+__global__ void k_topo_mark(int n_new, int n_old, const int32_t *__restrict__ old_inf,
+                            const int32_t *__restrict__ new_inf, int32_t *__restrict__ mark,
+                            ClearArgs A, double *__restrict__ u3x, double *__restrict__ u3y,
+                            double *__restrict__ u3z) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_new) return;
+  const bool was = s < n_old && old_inf[s] != 0;
+  const bool now = new_inf[s] != 0;
+  mark[s] = (!was && now) ? 1 : 0;
+  if (was && !now) {
+    for (int k = 0; k < ClearArgs::kArrays; ++k) A.a[k][s] = 0.;
+  } else if (!was && now) {
+    // u1, u2: entries 3..8 of the list (u, u1, u2, ...)
+    for (int k = 3; k < 9; ++k) A.a[k][s] = 0.;
+    if (s >= n_old) { u3x[s] = 0.; u3y[s] = 0.; u3z[s] = 0.; }
+  }
+}
+
+// pinsert's velocity of a new point (line.f90:86): the mean of the
+// velocities of the point it was inserted after and of that point's infront
+// at the time.  A point inserted after a new one continues a chain: after
+// pinsert the links read old -> new -> new ... -> old, each new point made
+// by its behind and inserted in front of the chain's end, so walking a
+// chain from its old start gives every value in the order pinsert set them.
+__global__ void k_topo_new_velocity(int n, const int32_t *__restrict__ mark,
+                                    const int32_t *__restrict__ inf,
+                                    const int32_t *__restrict__ beh, double *__restrict__ ux,
+                                    double *__restrict__ uy, double *__restrict__ uz) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !mark[p]) return;
+  const int b = beh[p] - 1;
+  if (mark[b]) return;  // not the start of a chain
+  int f = p;
+  for (int it = 0; it < n && mark[f]; ++it) f = inf[f] - 1;
+  const double uf[3] = {ux[f], uy[f], uz[f]};
+  double prev[3] = {ux[b], uy[b], uz[b]};
+  int q = p;
+  for (int it = 0; it < n && mark[q]; ++it) {
+    const double v[3] = {0.5 * (prev[0] + uf[0]), 0.5 * (prev[1] + uf[1]),
+                         0.5 * (prev[2] + uf[2])};
+    ux[q] = v[0]; uy[q] = v[1]; uz[q] = v[2];
+    prev[0] = v[0]; prev[1] = v[1]; prev[2] = v[2];
+    q = inf[q] - 1;
+  }
+}
+
 // remove_mean_pos (timestep.f90:86-92), first pass: per block the sums of
 // x and y over the active slots of its range (fixed order: a strided sum
 // per thread, then a tree over the block)
@@ -353,6 +405,26 @@ hipError_t launch_open_check(Ctx &c, int32_t *flag) {
   if (nopen == 0 || c.pcount == 0) return hipSuccess;
   hipLaunchKernelGGL(k_open_check, dim3(grid(c.pcount)), dim3(kBlock), 0, c.stream, c.pcount,
                      c.p.box_size, nopen, c.infront, c.x.x, c.x.y, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *new_inf,
+                                 const int32_t *new_beh, int32_t *mark) {
+  if (n_new <= 0) return hipSuccess;
+  ClearArgs A;
+  Vec3 vs[] = {c.u, c.u1, c.u2, c.uli, c.ubs, c.gi, c.gb, c.gii, c.gbb};
+  int k = 0;
+  for (const Vec3 &v : vs) {
+    A.a[k++] = v.x;
+    A.a[k++] = v.y;
+    A.a[k++] = v.z;
+  }
+  hipLaunchKernelGGL(k_topo_mark, dim3(grid(n_new)), dim3(kBlock), 0, c.stream, n_new, n_old,
+                     c.infront, new_inf, mark, A, c.u3.x, c.u3.y, c.u3.z);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_topo_new_velocity, dim3(grid(n_new)), dim3(kBlock), 0, c.stream, n_new,
+                     mark, new_inf, new_beh, c.u.x, c.u.y, c.u.z);
   return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ module qv_gpu
 
   public :: qv_create, qv_destroy, qv_set_params, qv_upload, qv_upload_history, &
             qv_velocity, qv_ab_step, qv_rk3_step, qv_enforce_periodic, qv_step, qv_download, &
-            qv_download_links, qv_boundary_removed, &
+            qv_download_links, qv_boundary_removed, qv_upload_topology, &
             qv_tree_dump, qv_closest, qv_probe_velocity, qv_vjp, qv_jvp, qv_get_timings, &
             qv_download_ghosts, qv_line_pinsert, qv_line_precon, qv_line_premove, &
             qv_host_alloc, qv_host_free, &
@@ -121,6 +121,17 @@ module qv_gpu
     integer(c_int) function qv_download(ctx, x, u, u1, u2, u3) bind(C, name='qv_download')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, x, u, u1, u2, u3
+    end function
+    !> positions and links after the host line operations, keeping the
+    !> device's velocities and AB history (new points and cleared slots get
+    !> pinsert's and clear_particle's values on the device)
+    integer(c_int) function qv_upload_topology(ctx, pcount, x, infront, behind) &
+        bind(C, name='qv_upload_topology')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: pcount
+      real(c_double), intent(in) :: x(3,*)
+      integer(c_int32_t), intent(in) :: infront(*), behind(*)
     end function
     !> the device links; they change on the device only when the boundary
     !> step removes loops at an open face

@@ -190,15 +190,9 @@ program qv_run
       call qv_check(qv_download_links(ctx, infront, behind), 'qv_download_links')
     t = t + dt
     if (mod(itime, shots) == 0) then
-      ! after the line operations the host holds the step's velocities (an
-      ! upload leaves none on the device); positions and history come back
-      if (line_ops) then
-        call qv_check(qv_download(ctx, c_loc(x), c_null_ptr, c_loc(u1), c_loc(u2), c_loc(u3)), &
-                      'qv_download')
-      else
-        call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
-                      'qv_download')
-      end if
+      ! the device holds the state, the line operations' changes included
+      call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
+                    'qv_download')
       call data_dump()
       call print_info(itime / shots)
       call printf(itime / shots)
@@ -738,22 +732,29 @@ contains
   subroutine line_step()
     type(qv_line_state) :: ls
     call ensure_capacity()
-    call qv_check(qv_download(ctx, c_loc(x), c_loc(u), c_loc(u1), c_loc(u2), c_loc(u3)), &
+    ! the positions and the ghosts the operations read; the velocities and
+    ! the AB history stay on the device, which gives new and cleared slots
+    ! their values (qv_upload_topology)
+    call qv_check(qv_download(ctx, c_loc(x), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), &
                   'qv_download')
-    call qv_check(qv_download_ghosts(ctx, c_loc(gi), c_loc(gb), c_loc(gii), c_loc(gbb)), &
-                  'qv_download_ghosts')
+    if (p%deriv_order == 4) then
+      call qv_check(qv_download_ghosts(ctx, c_loc(gi), c_loc(gb), c_loc(gii), c_loc(gbb)), &
+                    'qv_download_ghosts')
+    else
+      call qv_check(qv_download_ghosts(ctx, c_loc(gi), c_loc(gb), c_null_ptr, c_null_ptr), &
+                    'qv_download_ghosts')
+    end if
     ls = line_state()
     call qv_check(qv_line_pinsert(p, ls, counts), 'line.mod:pinsert')
     pcount = ls%pcount
+    call qv_check(qv_upload_topology(ctx, pcount, x, infront, behind), 'qv_upload_topology')
     if (mod(itime, recon_shots) == 0) then
-      call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
       call qv_check(qv_closest(ctx, closest, closestd), 'qv_closest')
       ls = line_state()
       if (.not. switch_off_recon) call qv_check(qv_line_precon(p, ls, counts), 'line.mod:precon')
       call qv_check(qv_line_premove(p, ls, counts), 'line.mod:premove')
+      call qv_check(qv_upload_topology(ctx, pcount, x, infront, behind), 'qv_upload_topology')
     end if
-    call qv_check(qv_upload(ctx, pcount, x, infront, behind), 'qv_upload')
-    call qv_check(qv_upload_history(ctx, c_loc(u1), c_loc(u2), c_loc(u3)), 'qv_upload_history')
   end subroutine
 
   !> setup_mesh (initial.f90:541-574) with its checks and messages, and the

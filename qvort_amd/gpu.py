@@ -31,7 +31,8 @@ QV_OK, QV_ERR_ARG, QV_ERR_HIP, QV_ERR_SINGULAR, QV_ERR_TREE_DEPTH, QV_ERR_STATE,
 QV_PROBE_GHOSTP, QV_PROBE_REFERENCE_SCALE, QV_PROBE_EMPTY_CORE = 1, 2, 4
 QV_FLAG_REMOVE_MEAN_POS = 1
 
-EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_history",
+EXPORTS = ["qv_create", "qv_destroy", "qv_set_params", "qv_upload", "qv_upload_topology",
+           "qv_upload_history",
            "qv_velocity", "qv_ab_step", "qv_rk3_step", "qv_enforce_periodic",
            "qv_boundary_removed", "qv_step", "qv_download", "qv_download_links",
            "qv_tree_dump", "qv_closest", "qv_probe_velocity", "qv_vjp", "qv_jvp", "qv_set_stream", "qv_velocity_device",
@@ -124,6 +125,7 @@ def lib():
         L.qv_set_params.argtypes = [vp, P(QvParams)]
         L.qv_upload.argtypes = [vp, C.c_int32, d, i32, i32]
         L.qv_upload_history.argtypes = [vp, d, d, d]
+        L.qv_upload_topology.argtypes = [vp, C.c_int32, d, i32, i32]
         L.qv_velocity.argtypes = [vp, d, d, d, i64]
         L.qv_ab_step.argtypes = [vp]
         L.qv_rk3_step.argtypes = [vp]
@@ -225,6 +227,19 @@ class Context:
 
     def upload_filaments(self, fil) -> None:
         self.upload(fil.x, fil.infront, fil.behind)
+
+    def upload_topology(self, x, infront, behind) -> None:
+        """Positions and links after host line operations, keeping the
+        device's velocities and AB history (qv_upload_topology)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        inf = np.ascontiguousarray(infront, dtype=np.int32)
+        beh = np.ascontiguousarray(behind, dtype=np.int32)
+        n = inf.shape[0]
+        if x.shape != (n, 3) or beh.shape != (n,):
+            raise ValueError("x must be (pcount,3), links (pcount,)")
+        _check(lib().qv_upload_topology(self._h, n, _dp(x), _ip(inf), _ip(beh)),
+               "qv_upload_topology")
+        self.pcount = n
 
     def upload_history(self, u1=None, u2=None, u3=None) -> None:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)

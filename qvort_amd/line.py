@@ -119,16 +119,19 @@ def premove(p: gpu.QvParams, st: LineState, counts: gpu.QvLineCounts) -> None:
 def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
                   counts: gpu.QvLineCounts, *, search: bool) -> None:
     """The line operations after a step's velocity and AB update
-    (run.f90:60-72): the state and the step's ghosts come back from the
+    (run.f90:60-72): the positions and the step's ghosts come back from the
     device, pinsert runs on the host, and when `search` is set (recon_shots
     divides itime) the device searches the step's tree for every slot's
-    closest point before precon and premove run on the host.  The result is
-    uploaded with its AB history; the caller then runs the boundary step
-    (qv_enforce_periodic) as the reference does after its ghostp.  The host
-    owns the topology: `st` holds the current links (and closest/closestd)
-    between calls; the device sends back positions, velocities and ghosts,
-    and in openx / openxy boxes the links, which its boundary step changes
-    when a loop leaves through an open face."""
+    closest point before precon and premove run on the host.  Positions and
+    links go back with qv_upload_topology after pinsert and after premove,
+    and the device gives new and cleared slots their velocities and AB
+    history; the caller then runs the boundary step (qv_enforce_periodic) as
+    the reference does after its ghostp.  The host owns the topology: `st`
+    holds the current links (and closest/closestd) between calls; its
+    velocity and history arrays are not kept up to date (the device's are
+    the state); in openx / openxy boxes the links come back first, since the
+    device's boundary step changes them when a loop leaves through an open
+    face."""
     n = ctx.pcount
     if n > st.capacity:
         raise ValueError("the line state holds fewer slots than the device")
@@ -136,15 +139,22 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
     if p.boundary in (gpu.BC["openx"], gpu.BC["openxy"]):
         # the device's boundary step may have removed loops at an open face
         ctx.download_links(st._infront, st._behind)
-    # straight into the state's arrays (their first n rows are contiguous)
-    ctx.download(out=[getattr(st, "_" + k)[:n] for k in ("x", "u", "u1", "u2", "u3")])
-    ctx.download_ghosts(out=[getattr(st, "_" + k)[:n]
-                             for k in ("ghosti", "ghostb", "ghostii", "ghostbb")])
+    # straight into the state's arrays (their first n rows are contiguous):
+    # the positions and the ghosts the operations read (ghostii and ghostbb
+    # only for fourth-order derivatives); velocities and AB history stay on
+    # the device, which applies pinsert's and clear_particle's values to
+    # them (qv_upload_topology), so the host's copies are not read
+    _dp = gpu._dp
+    gpu._check(gpu.lib().qv_download(ctx._h, _dp(st._x[:n]), None, None, None, None),
+               "qv_download")
+    g = [st._ghosti[:n], st._ghostb[:n]]
+    g += [st._ghostii[:n], st._ghostbb[:n]] if p.deriv_order == 4 else [None, None]
+    gpu._check(gpu.lib().qv_download_ghosts(ctx._h, *[None if a is None else _dp(a) for a in g]),
+               "qv_download_ghosts")
     pinsert(p, st, counts)
+    ctx.upload_topology(st.x, st.infront, st.behind)
     if search:
-        ctx.upload(st.x, st.infront, st.behind)
         ctx.closest(st.closest, st.closestd, inplace=True)
         precon(p, st, counts)
         premove(p, st, counts)
-    ctx.upload(st.x, st.infront, st.behind)
-    ctx.upload_history(st.u1, st.u2, st.u3)
+        ctx.upload_topology(st.x, st.infront, st.behind)

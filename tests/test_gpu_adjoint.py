@@ -137,3 +137,53 @@ def test_multistep_gradient_against_differences(objective):
             assert abs(g[i, a] - fd) <= 1e-6 * max(abs(fd), floor)
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("objective", ["length", "energy"])
+def test_kelvin_8192_200_steps_gradient(objective):
+    """The adjoint configuration at its size: a periodic line of 8192 points
+    along z (L = 0.1 cm) with 16 Kelvin waves of amplitude ~N(0, (2e-4 cm)^2),
+    200 AB steps checkpointed every 20, the gradient of the objective against
+    central differences with h = 1e-7 cm (Richardson-extrapolated over h and
+    2h, as above) for the largest component and two others, to 1e-6; and the
+    mixed-precision gradient (FP32 pair terms) within 1e-5 of the FP64 one
+    relative to its largest component."""
+    import time
+
+    from qvort_amd import adjoint as AD
+    from qvort_amd import geometry as G
+    fil = G.kelvin_line(8192, 0.1, amp_sigma=2e-4, seed=20261202)
+    eng = AD.Engine(fil, box_size=0.1, boundary="periodic")
+    eng32 = AD.Engine(fil, box_size=0.1, boundary="periodic", precision="fp32c")
+    try:
+        dt = 0.25 * AD.dt_max(0.1 / 8192)
+        nsteps = 200
+        t0 = time.perf_counter()
+        J, g = AD.gradient(objective, fil.x, eng, nsteps, dt, checkpoint_every=20)
+        print(objective, "J", J, "dt", dt, f"gradient {time.perf_counter() - t0:.1f} s",
+              flush=True)
+        scale = np.max(np.abs(g))
+        _, g32 = AD.gradient(objective, fil.x, eng32, nsteps, dt, checkpoint_every=20)
+        d32 = np.max(np.abs(g32 - g)) / scale
+        print("fp32c gradient vs fp64:", d32, flush=True)
+        assert d32 <= 1e-5
+        top = int(np.argmax(np.abs(g).ravel()))
+        rng = np.random.Generator(np.random.PCG64(8))
+        comps = [(top // 3, top % 3, True)]
+        comps += [(int(i), 0, False) for i in rng.choice(fil.pcount, size=2, replace=False)]
+        for i, a, strict in comps:
+            d = []
+            for h in (H, 2 * H):
+                xp, xm = fil.x.copy(), fil.x.copy()
+                xp[i, a] += h
+                xm[i, a] -= h
+                d.append((AD.objective_value(objective, xp, eng, nsteps, dt) -
+                          AD.objective_value(objective, xm, eng, nsteps, dt)) / (2 * h))
+            fd = (4 * d[0] - d[1]) / 3
+            print(i, a, strict, g[i, a], fd, d[0], f"{time.perf_counter() - t0:.1f} s",
+                  flush=True)
+            floor = 0.0 if strict else 1e-2 * scale
+            assert abs(g[i, a] - fd) <= 1e-6 * max(abs(fd), floor)
+    finally:
+        eng.close()
+        eng32.close()

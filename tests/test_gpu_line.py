@@ -126,3 +126,57 @@ def test_pinned_host_memory_round_trip():
     finally:
         assert L.qv_host_free(ptr) == 0
     assert L.qv_host_free(None) == 0
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_topology_upload_keeps_device_history(name):
+    """qv_upload_topology after pinsert and after premove: the device's
+    velocities and AB history equal the reference's after those operations
+    (new points: u the mean of their neighbours' velocities in pinsert's
+    order, u1 = u2 = 0; cleared slots: zeros; the rest untouched) with the
+    host's copies of them never sent."""
+    from qvort_amd import gpu, line
+    g = load_golden(name)
+    meta = g["meta"]
+    pr = dict(meta["params"])
+    pr.pop("recon_shots")
+    p = gpu.params(**pr)
+    print("case", name, flush=True)
+    st = line.LineState.from_arrays(capacity=2 * meta["pcount"] + 8, x=g["x0"],
+                                    infront=g["infront"], behind=g["behind"])
+    c = line.new_counts()
+
+    def check(ctx, stage):
+        n = int(g[f"{stage}__pcount"])
+        _, u, u1, u2, _ = ctx.download()
+        assert ctx.pcount == n
+        act = g[f"{stage}__infront"] != 0
+        for got, key in ((u, "u"), (u1, "u1"), (u2, "u2")):
+            ref = g[f"{stage}__{key}"]
+            assert not np.any(got[~act]), (stage, key)
+            nr = np.linalg.norm(ref[act], axis=1)
+            z = nr == 0
+            assert not np.any(got[act][z]), (stage, key)
+            e = np.linalg.norm(got[act][~z] - ref[act][~z], axis=1) / nr[~z]
+            assert e.size == 0 or e.max() <= 1e-12, (stage, key, e.max())
+
+    with gpu.Context(p) as ctx:
+        ctx.upload(g["x0"], g["infront"], g["behind"])
+        ctx.velocity_only()
+        ctx.ab_step()
+        n = ctx.pcount
+        st.pcount = n
+        gpu._check(gpu.lib().qv_download(ctx._h, gpu._dp(st._x[:n]), None, None, None, None),
+                   "qv_download")
+        ctx.download_ghosts(out=[getattr(st, "_" + k)[:n]
+                                 for k in ("ghosti", "ghostb", "ghostii", "ghostbb")])
+        line.pinsert(p, st, c)
+        ctx.upload_topology(st.x, st.infront, st.behind)
+        check(ctx, "pinsert")
+        if "premove__pcount" not in g:
+            return  # recon_shots does not divide the first step
+        ctx.closest(st.closest, st.closestd, inplace=True)
+        line.precon(p, st, c)
+        line.premove(p, st, c)
+        ctx.upload_topology(st.x, st.infront, st.behind)
+        check(ctx, "premove")
