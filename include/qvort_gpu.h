@@ -247,8 +247,8 @@ int qv_jvp_device(qv_ctx *ctx, const double *d_x, const double *d_v, double *d_j
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc): the
  * scratch of the adjoint kernels is a static device array of the library
- * (30 doubles per slot, at most 65536 slots: 15.7 MB per device, held until
- * the process ends) instead of buffers qv_vjp_device allocates on its first
+ * (5 Mi doubles, 42 MB per device, held until the process ends: at most
+ * 65536 slots) instead of buffers qv_vjp_device allocates on its first
  * call.  Same arguments, same result bit for bit.  More than 65536 slots:
  * QV_ERR_SCRATCH.  Every call on one device shares that device's array:
  * calls that overlap on one GPU (two contexts on different streams, or two

@@ -126,9 +126,14 @@ struct Tile {
 };
 
 // pass 1: one thread per target i, sum over sources and shells of
-// -(dL/dR1 + dL/dR2)
-__global__ void __launch_bounds__(kBlock) k_adj_target(AdjArgs A, Shells S) {
+// -(dL/dR1 + dL/dR2); the sources of chunk blockIdx.y only (the source
+// range is cut into chunks when the targets alone do not fill the GPU):
+// the sums go to part (3 per target per chunk), or straight to the slot's
+// scratch when there is one chunk
+__global__ void __launch_bounds__(kBlock) k_adj_target(AdjArgs A, Shells S, int len,
+                                                       double *__restrict__ part) {
   __shared__ Tile t;
+  const int lo = blockIdx.y * len, hi = min(A.n, lo + len);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < A.n && A.inf[i] != 0;
   double xi[3] = {0., 0., 0.}, w[3] = {0., 0., 0.};
@@ -139,24 +144,24 @@ __global__ void __launch_bounds__(kBlock) k_adj_target(AdjArgs A, Shells S) {
     bi = A.beh[i] - 1;
   }
   double acc[3] = {0., 0., 0.};
-  for (int sh = 0; sh < S.n; ++sh) {
-    const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    for (int base = 0; base < A.n; base += kTile) {
-      __syncthreads();
-      {
-        const int j = base + threadIdx.x;
-        const bool in = j < A.n;
-        t.x[threadIdx.x] = in ? A.X[j] : 0.;
-        t.y[threadIdx.x] = in ? A.Y[j] : 0.;
-        t.z[threadIdx.x] = in ? A.Z[j] : 0.;
-        t.a[threadIdx.x] = in ? A.GX[j] : 0.;
-        t.b[threadIdx.x] = in ? A.GY[j] : 0.;
-        t.c[threadIdx.x] = in ? A.GZ[j] : 0.;
-        t.k0[threadIdx.x] = in ? (A.inf[j] != 0) : 0;
-      }
-      __syncthreads();
-      if (!act) continue;
-      const int m = min(kTile, A.n - base);
+  for (int base = lo; base < hi; base += kTile) {
+    __syncthreads();
+    {
+      const int j = base + threadIdx.x;
+      const bool in = j < hi;
+      t.x[threadIdx.x] = in ? A.X[j] : 0.;
+      t.y[threadIdx.x] = in ? A.Y[j] : 0.;
+      t.z[threadIdx.x] = in ? A.Z[j] : 0.;
+      t.a[threadIdx.x] = in ? A.GX[j] : 0.;
+      t.b[threadIdx.x] = in ? A.GY[j] : 0.;
+      t.c[threadIdx.x] = in ? A.GZ[j] : 0.;
+      t.k0[threadIdx.x] = in ? (A.inf[j] != 0) : 0;
+    }
+    __syncthreads();
+    if (!act) continue;
+    const int m = min(kTile, hi - base);
+    for (int sh = 0; sh < S.n; ++sh) {
+      const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
       for (int jj = 0; jj < m; ++jj) {
         const int j = base + jj;
         if (!t.k0[jj] || j == bi || (sh == 0 && j == i)) continue;
@@ -172,14 +177,17 @@ __global__ void __launch_bounds__(kBlock) k_adj_target(AdjArgs A, Shells S) {
     }
   }
   if (i >= A.n) return;
-  double *o = slot_out(A, i);
+  double *o = part ? part + 3 * ((int64_t)blockIdx.y * A.n + i) : slot_out(A, i);
   for (int k = 0; k < 3; ++k) o[k] = acc[k];
 }
 
 // pass 2: one thread per source j, sum over targets and shells of dL/dR1
-// (at x_j) and dL/dR2 (at g_j)
-__global__ void __launch_bounds__(kBlock) k_adj_source(AdjArgs A, Shells S) {
+// (at x_j) and dL/dR2 (at g_j); the targets of chunk blockIdx.y only: the
+// sums go to part (6 per source per chunk) or straight to the scratch
+__global__ void __launch_bounds__(kBlock) k_adj_source(AdjArgs A, Shells S, int len,
+                                                       double *__restrict__ part) {
   __shared__ Tile t;
+  const int lo = blockIdx.y * len, hi = min(A.n, lo + len);
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = j < A.n && A.inf[j] != 0;
   double xj[3] = {0., 0., 0.}, gj[3] = {0., 0., 0.};
@@ -188,25 +196,25 @@ __global__ void __launch_bounds__(kBlock) k_adj_source(AdjArgs A, Shells S) {
     gj[0] = A.GX[j]; gj[1] = A.GY[j]; gj[2] = A.GZ[j];
   }
   double a1[3] = {0., 0., 0.}, a2[3] = {0., 0., 0.};
-  for (int sh = 0; sh < S.n; ++sh) {
-    const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
-    for (int base = 0; base < A.n; base += kTile) {
-      __syncthreads();
-      {
-        const int i = base + threadIdx.x;
-        const bool in = i < A.n;
-        t.x[threadIdx.x] = in ? A.X[i] : 0.;
-        t.y[threadIdx.x] = in ? A.Y[i] : 0.;
-        t.z[threadIdx.x] = in ? A.Z[i] : 0.;
-        t.a[threadIdx.x] = in ? A.wx[i] : 0.;
-        t.b[threadIdx.x] = in ? A.wy[i] : 0.;
-        t.c[threadIdx.x] = in ? A.wz[i] : 0.;
-        t.k0[threadIdx.x] = in ? (A.inf[i] != 0) : 0;
-        t.k1[threadIdx.x] = in ? A.beh[i] - 1 : -1;
-      }
-      __syncthreads();
-      if (!act) continue;
-      const int m = min(kTile, A.n - base);
+  for (int base = lo; base < hi; base += kTile) {
+    __syncthreads();
+    {
+      const int i = base + threadIdx.x;
+      const bool in = i < hi;
+      t.x[threadIdx.x] = in ? A.X[i] : 0.;
+      t.y[threadIdx.x] = in ? A.Y[i] : 0.;
+      t.z[threadIdx.x] = in ? A.Z[i] : 0.;
+      t.a[threadIdx.x] = in ? A.wx[i] : 0.;
+      t.b[threadIdx.x] = in ? A.wy[i] : 0.;
+      t.c[threadIdx.x] = in ? A.wz[i] : 0.;
+      t.k0[threadIdx.x] = in ? (A.inf[i] != 0) : 0;
+      t.k1[threadIdx.x] = in ? A.beh[i] - 1 : -1;
+    }
+    __syncthreads();
+    if (!act) continue;
+    const int m = min(kTile, hi - base);
+    for (int sh = 0; sh < S.n; ++sh) {
+      const double s[3] = {S.s[sh][0], S.s[sh][1], S.s[sh][2]};
       for (int ii = 0; ii < m; ++ii) {
         const int i = base + ii;
         if (!t.k0[ii] || j == t.k1[ii] || (sh == 0 && j == i)) continue;
@@ -227,11 +235,36 @@ __global__ void __launch_bounds__(kBlock) k_adj_source(AdjArgs A, Shells S) {
     }
   }
   if (j >= A.n) return;
-  double *o = slot_out(A, j);
-  for (int k = 0; k < 3; ++k) {
-    o[3 + k] = a1[k];
-    o[6 + k] = a2[k];
+  if (part) {
+    double *o = part + 6 * ((int64_t)blockIdx.y * A.n + j);
+    for (int k = 0; k < 3; ++k) {
+      o[k] = a1[k];
+      o[3 + k] = a2[k];
+    }
+  } else {
+    double *o = slot_out(A, j);
+    for (int k = 0; k < 3; ++k) {
+      o[3 + k] = a1[k];
+      o[6 + k] = a2[k];
+    }
   }
+}
+
+// the chunks' sums of both passes in chunk order into the slot's scratch
+__global__ void __launch_bounds__(kBlock) k_adj_chunks(AdjArgs A, int nchunk,
+                                                       const double *__restrict__ pt,
+                                                       const double *__restrict__ ps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double s[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int c = 0; c < nchunk; ++c) {
+    const double *a = pt + 3 * ((int64_t)c * A.n + i);
+    const double *b = ps + 6 * ((int64_t)c * A.n + i);
+    for (int k = 0; k < 3; ++k) s[k] = s[k] + a[k];
+    for (int k = 0; k < 6; ++k) s[3 + k] = s[3 + k] + b[k];
+  }
+  double *o = slot_out(A, i);
+  for (int k = 0; k < 9; ++k) o[k] = s[k];
 }
 
 // local term: w . d u_LI / d(coordinate) for each of the inputs x, gi, gb
@@ -415,12 +448,32 @@ AdjArgs make_args(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch) {
 
 }  // namespace
 
-hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch) {
+int vjp_chunks(int n, int &len) {
+  // about 4096 wavefronts, 16 per CU, in whole tiles
+  const int waves = grid(n) * (kBlock / 64);
+  const int want = (4096 + waves - 1) / waves;
+  len = (n + want - 1) / want;
+  len = ((len + kTile - 1) / kTile) * kTile;
+  return (n + len - 1) / len;
+}
+
+hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch, double *part,
+                      int64_t part_doubles) {
   AdjArgs A = make_args(c, w, r, mixed, scratch);
   Shells S = make_shells(c.p);
   const int g = grid(c.pcount);
-  hipLaunchKernelGGL(k_adj_target, dim3(g), dim3(kBlock), 0, c.stream, A, S);
-  hipLaunchKernelGGL(k_adj_source, dim3(g), dim3(kBlock), 0, c.stream, A, S);
+  int len = 0;
+  int nchunk = vjp_chunks(c.pcount, len);
+  if (!part || 9 * (int64_t)nchunk * c.pcount > part_doubles) {
+    nchunk = 1;  // no room for partial sums: one chunk, straight to the scratch
+    len = c.pcount;
+    part = nullptr;
+  }
+  double *pt = part, *ps = part ? part + 3 * (int64_t)nchunk * c.pcount : nullptr;
+  hipLaunchKernelGGL(k_adj_target, dim3(g, nchunk), dim3(kBlock), 0, c.stream, A, S, len, pt);
+  hipLaunchKernelGGL(k_adj_source, dim3(g, nchunk), dim3(kBlock), 0, c.stream, A, S, len, ps);
+  if (part)
+    hipLaunchKernelGGL(k_adj_chunks, dim3(g), dim3(kBlock), 0, c.stream, A, nchunk, pt, ps);
   hipLaunchKernelGGL(k_adj_lia, dim3(g), dim3(kBlock), 0, c.stream, A);
   hipLaunchKernelGGL(k_adj_combine, dim3(g), dim3(kBlock), 0, c.stream, A);
   return hipGetLastError();

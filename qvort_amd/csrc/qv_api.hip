@@ -13,14 +13,16 @@ namespace qv {
 namespace {
 
 // Scratch of qv_vjp_device_noalloc: per slot the cotangent and the result
-// (3 + 3 doubles, SoA) and the adjoint kernels' 24 partial sums, for up to
-// kNoAllocSlots slots (15.7 MB).  Each device the library runs on holds its
-// own copy until the process ends.  In host code `scratch` names a
+// (3 + 3 doubles, SoA) and the adjoint kernels' 24 partial sums, then the
+// partial sums of their source / target chunks (9 doubles per slot per
+// chunk, as qv_vjp_device uses them), for up to kNoAllocSlots slots: 5 Mi
+// doubles (42 MB) hold that for every slot count up to the limit.  Each
+// device the library runs on holds its own copy until the process ends.  In host code `scratch` names a
 // host-side stand-in, not a device address: its address on a device comes
 // from hipGetSymbolAddress with that device current (scratch_setup, called
 // by qv_create) and only that address is passed to kernels.
 constexpr int32_t kNoAllocSlots = 65536;
-constexpr size_t kScratchDoubles = (size_t)30 * kNoAllocSlots;
+constexpr size_t kScratchDoubles = (size_t)5 << 20;
 __device__ __align__(256) double scratch[kScratchDoubles];
 
 constexpr int kMaxDevices = 64;
@@ -183,6 +185,13 @@ int velocity_stage_launch(Ctx &c) {
   if (c.p.velocity == QV_VEL_TREE) {
     if (c.nact > 0) TRY(launch_walk(c));
   } else {
+    const int64_t need =
+        6 * (int64_t)direct_chunks(c.pcount, c.p.precision == QV_FP32C) * c.pcount;
+    if (need > c.direct_part_doubles) {
+      TRY(hipStreamSynchronize(c.stream));
+      TRY(dalloc(c.direct_part, (size_t)need));
+      c.direct_part_doubles = need;
+    }
     TRY(launch_direct(c));
   }
   TRY(hipEventRecord(c.ev[3], c.stream));
@@ -293,6 +302,13 @@ void dump_rec(DumpCtx &D, int64_t n) {
 }
 
 int adj_reserve(Ctx &c) {
+  int len = 0;
+  const int64_t need = 9 * (int64_t)vjp_chunks(c.pcount, len) * c.pcount;
+  if (need > c.adj_part_doubles) {
+    TRY(hipStreamSynchronize(c.stream));
+    TRY(dalloc(c.adj_part, (size_t)need));
+    c.adj_part_doubles = need;
+  }
   if (c.adj_cap >= c.pcount) return QV_OK;
   TRY(dalloc(c.adj, (size_t)24 * c.pcount));
   TRY(valloc(c.aw, c.pcount));
@@ -310,7 +326,8 @@ int derivative(Ctx &c, const double *d_in, double *d_out, bool vjp) {
   if (rc) return rc;
   TRY(launch_ghost_lia(c));
   TRY(launch_aos_to_soa(c, d_in, c.aw));
-  if (vjp) TRY(launch_vjp(c, c.aw, c.ar, c.p.precision == QV_FP32C ? 1 : 0, c.adj));
+  if (vjp) TRY(launch_vjp(c, c.aw, c.ar, c.p.precision == QV_FP32C ? 1 : 0, c.adj, c.adj_part,
+                          c.adj_part_doubles));
   else TRY(launch_jvp(c, c.aw, c.ar));
   TRY(launch_soa_to_aos(c, c.ar, d_out));
   return QV_OK;
@@ -450,7 +467,7 @@ int qv_destroy(qv_ctx *ctx) {
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
                 c->slot_s, c->walk_order, c->walk_part, c->nnode_of, c->node_off, c->sort_tmp,
-                c->counters, c->evals, c->sub_list,
+                c->counters, c->evals, c->sub_list, c->direct_part, c->adj_part,
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
@@ -855,19 +872,23 @@ int qv_vjp_device_noalloc(qv_ctx *ctx, const double *d_x, const double *d_w,
   if (!ctx || !d_w || !d_xbar) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (c->pcount == 0) return QV_ERR_STATE;
-  if (c->pcount > kNoAllocSlots) return QV_ERR_SCRATCH;
+  int len = 0;
+  const int64_t part = 9 * (int64_t)vjp_chunks(c->pcount, len) * c->pcount;
+  if (c->pcount > kNoAllocSlots || 30 * (int64_t)c->pcount + part > (int64_t)kScratchDoubles)
+    return QV_ERR_SCRATCH;
   if (c->p.velocity != QV_VEL_BS || !c->links_consistent) return QV_ERR_ARG;
   double *base = scratch_of(c->device);
   if (!base) return QV_ERR_STATE;
   const size_t n = (size_t)c->pcount;
   const Vec3 w{base, base + n, base + 2 * n};
   const Vec3 r{base + 3 * n, base + 4 * n, base + 5 * n};
-  double *adj = base + 6 * n;  // 24 n doubles
+  double *adj = base + 6 * n;  // 24 n doubles, then the chunks' partial sums
   int rc = set_positions_device(*c, d_x);
   if (rc) return rc;
   TRY(launch_ghost_lia(*c));
   TRY(launch_aos_to_soa(*c, d_w, w));
-  TRY(launch_vjp(*c, w, r, c->p.precision == QV_FP32C ? 1 : 0, adj));
+  TRY(launch_vjp(*c, w, r, c->p.precision == QV_FP32C ? 1 : 0, adj, base + 30 * n,
+                 (int64_t)kScratchDoubles - 30 * (int64_t)n));
   TRY(launch_soa_to_aos(*c, r, d_xbar));
   return QV_OK;
 }

@@ -121,6 +121,8 @@ struct Ctx {
   int32_t *sub_list = nullptr;   // roots of the slot-order moment subtrees
   int32_t *big_buf = nullptr;    // big-node list, chunk bases, chunk owners
   double *partial = nullptr;     // per-chunk partial sums of the big nodes
+  double *direct_part = nullptr;  // FP64 direct sum: 6 sums per slot per chunk
+  int64_t direct_part_doubles = 0;
   int64_t big_cap_ints = 0, partial_cap = 0;
   int64_t *evals = nullptr;
   Tree tree;
@@ -130,6 +132,8 @@ struct Ctx {
   qv_timings timings{};
   // adjoint: 24 doubles of scratch per slot, tangent/cotangent and result
   double *adj = nullptr;
+  double *adj_part = nullptr;  // partial sums of the VJP's chunks
+  int64_t adj_part_doubles = 0;
   Vec3 aw, ar;
   int32_t adj_cap = 0;
   bool links_consistent = false;  // infront(behind(i)) == i for every point
@@ -196,6 +200,9 @@ hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *ne
 // clear_particle's zeroes in the m slots of d_list (device)
 hipError_t launch_clear_slots(Ctx &c, int m, const int32_t *d_list);
 hipError_t launch_direct(Ctx &c);
+// source chunks of the direct sum at pcount slots (6 pcount doubles of
+// partial sums each)
+int direct_chunks(int pcount, bool fp32);
 // 21-bit Hilbert-curve keys of n points (x[i*stride], y[..], z[..]) on a
 // 128^3 grid of the box, idx[i] = i; with inf, points with inf[i] = 0 get
 // the key 2^21
@@ -214,8 +221,12 @@ int hilbert_order(Ctx &c, int n, Vec3 pos, const int32_t *inf, int32_t *order, h
 int join_aux(Ctx &c);
 int tree_build(Ctx &c);          // returns qv_status
 hipError_t launch_walk(Ctx &c);
-// scratch: 24 doubles per slot, every one written before it is read
-hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch);
+// scratch: 24 doubles per slot, every one written before it is read;
+// part (may be NULL): room for the partial sums of source / target chunks,
+// 9 doubles per slot per chunk (vjp_chunks); without room, one chunk
+hipError_t launch_vjp(Ctx &c, Vec3 w, Vec3 r, int mixed, double *scratch, double *part,
+                      int64_t part_doubles);
+int vjp_chunks(int n, int &len);
 hipError_t launch_jvp(Ctx &c, Vec3 v, Vec3 r);
 hipError_t launch_closest(Ctx &c, bool tree);
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,

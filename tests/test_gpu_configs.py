@@ -13,8 +13,8 @@
   1e-12 of the oracle at every point, FP32C within 1e-5 of FP64, equal
   interaction counts.
 * The ensemble of 256 rings (points U{200..5000}, open box), 500 AB steps
-  from captured HIP graphs: FP64 per system equal to the single-system path
-  (a sample of systems), FP32C velocity within 1e-5 of FP64, graph replay
+  from captured HIP graphs: FP64 per system within 1e-12 of the
+  single-system path (a sample of systems), FP32C velocity within 1e-5 of FP64, graph replay
   equal to eager steps, every state finite after 500 steps.
 """
 import numpy as np
@@ -178,7 +178,9 @@ def test_ensemble_256_rings_500_steps():
         with gpu.Context(gpu.params(precision="fp64", **kw)) as c:
             c.upload_filaments(systems[s])
             u1 = c.velocity()[0]
-        assert np.array_equal(u1, u64[s])
+        act = systems[s].infront != 0
+        e = np.linalg.norm(u1[act] - u64[s][act], axis=1) / np.linalg.norm(u64[s][act], axis=1)
+        assert e.max() <= 1e-12
     with gpu.Batch(gpu.params(precision="fp32c", **kw), systems) as b:
         b.tune(2)
         u32 = b.velocity()

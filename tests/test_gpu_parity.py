@@ -3,8 +3,8 @@ and against the oracle, through the C ABI.
 
 Bars (FP64 path): tree occupancy, cell bounds, leaf indices and the eval
 counter identical; the moments of every node of at most 64 points
-identical; direct (BS) sums bit-identical; tree velocities within 1e-12
-relative per point; AB trajectories within 1e-15 absolute.  FP32C
+identical; direct (BS) and tree velocities within 1e-12 relative per
+point; AB trajectories within 1e-15 absolute.  FP32C
 path: within 1e-5 relative per point of the FP64 reference.
 """
 import numpy as np
@@ -37,8 +37,11 @@ def test_fp64_velocity_vs_golden(name):
     act = g["infront"] != 0
     assert np.max(rel_err(uli, g["u_li"], act)) <= 1e-14
     if meta["params"]["velocity"] == "BS":
-        # direct sum: same order of operations as biot_savart(_shift)
-        assert np.array_equal(ubs, g["u_bs"])
+        # direct sum: the reference's pair term with refined reciprocals,
+        # source chunks summed in a fixed order
+        scale = np.linalg.norm(g["u"][act], axis=1)
+        err = np.linalg.norm(ubs[act] - g["u_bs"][act], axis=1) / scale
+        assert np.max(err) <= 1e-12
     else:
         assert ev == int(g["evals"][0])
         # empty slots of the reference's arrays hold whatever was there
