@@ -138,7 +138,6 @@ int reserve(Ctx &c, int32_t n, bool keep = false) {
   TRY(dalloc(c.walk_part, 3 * cap));
   TRY(dalloc(c.nnode_of, cap + 1));
   TRY(dalloc(c.node_off, cap + 1));
-  TRY(dalloc(c.sub_list, cap / 2 + 2));
   c.cap = (int32_t)cap;
   return QV_OK;
 }
@@ -467,7 +466,7 @@ int qv_destroy(qv_ctx *ctx) {
   for (Vec3 *v : vs) vfree(*v);
   void *ps[] = {c->infront, c->behind, c->stage, c->key0, c->key1, c->key0_s, c->slot,
                 c->slot_s, c->walk_order, c->walk_part, c->nnode_of, c->node_off, c->sort_tmp,
-                c->counters, c->evals, c->sub_list, c->direct_part, c->adj_part,
+                c->counters, c->evals, c->direct_part, c->adj_part,
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,

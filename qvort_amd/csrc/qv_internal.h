@@ -117,8 +117,7 @@ struct Ctx {
   Vec3 xs, gs;  // positions and ghosts in sorted order
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  int32_t *counters = nullptr;   // [1] error flags, [2] big nodes, [3] chunks, [4] subtrees
-  int32_t *sub_list = nullptr;   // roots of the slot-order moment subtrees
+  int32_t *counters = nullptr;   // [1] error flags, [2] big nodes, [3] chunks, [6] open faces
   int32_t *big_buf = nullptr;    // big-node list, chunk bases, chunk owners
   double *partial = nullptr;     // per-chunk partial sums of the big nodes
   double *direct_part = nullptr;  // FP64 direct sum: 6 sums per slot per chunk

@@ -25,17 +25,17 @@
 // sum ((circ + g) - x), C = sum x (g - x)^T - cent circ^T.  The reference
 // sums each node's members in slot order (a child's member list is a stable
 // filter of its parent's, and the root's is f(:) itself).  Every node of at
-// most 64 members does the same here, so its moments are the reference's
-// bit for bit: a wavefront takes a subtree of at most 64 members, sorts its
-// members by slot across the lanes and lets each lane fold one node's
-// members in that order.  Larger nodes (the top levels: about 1% of the
-// nodes at 1M points) are summed in chunks in parallel and differ from the
-// reference's sums in their last bits.  circ is a sum of segment vectors
-// that cancels over every closed loop inside a node, so its rounding is a
-// large share of the value, and the cell term of a node carries it to the
-// velocity: at 1M points, sums in key order moved the worst target's
-// velocity by 1.4e-12 relative, slot order for the nodes up to 64 members
-// leaves 1.2e-13.
+// most 8 members does the same here, so its moments are the reference's bit
+// for bit: one thread sorts the node's member slots in registers (a sorting
+// network) and folds the members in that order.  Larger nodes are summed in
+// key order (up to 64 members, one thread each) or in chunks in parallel,
+// and differ from the reference's sums in their last bits.  circ is a sum
+// of segment vectors that cancels over every closed loop inside a node, so
+// its rounding is a large share of the value, and the cell term of a node
+// carries it to the velocity: at 1M points, sums in key order moved the
+// worst target's velocity by 1.4e-12 relative; slot order for the nodes of
+// up to 8 members brings the worst point to about 2e-13 (a CPU emulation of
+// the rounding: 1.6e-13 with 8, 1.2e-13 with 64 on 50,000 targets).
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -52,6 +52,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kSmallNode = 64;  // nodes up to this size: one thread each
+constexpr int kTinyNode = 8;    // nodes up to this size: summed in slot order
 
 __host__ __device__ inline int grid_of(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -298,14 +299,10 @@ struct MomArgs {
   float *mp32;
   int32_t *big_list;   // big node ids (count > kSmallNode), in discovery order
   int32_t *big_base;   // first chunk of each big node
-  int32_t *big_count;  // [0] big nodes, [1] chunks, [2] subtrees
+  int32_t *big_count;  // [0] big nodes, [1] chunks
   int32_t *chunk_big;  // big-list index of each chunk
   double *partial;     // 15 sums per chunk
   const int32_t *slots;  // slot of each sorted position
-  // roots of the subtrees whose nodes are summed in slot order: the nodes of
-  // 2..kSmallNode members whose parent is bigger (or the root, when it
-  // holds at most kSmallNode points); at most nact / 2 of them
-  int32_t *sub_list;
 };
 
 constexpr int kChunk = 4096;  // members per workgroup in the big-node sums
@@ -438,35 +435,50 @@ __device__ inline void add_point(const MomArgs &A, int t, double s[15]) {
   add_member(x, g, s);
 }
 
-// one thread per node: leaves and empty records here; nodes of
-// 2..kSmallNode members are summed by k_moments_subtree, bigger ones are
-// queued for k_moments_chunks / k_moments_big and list their children of
-// 2..kSmallNode members as subtrees
+// exclusive prefix of v over the wavefront's lanes, and one atomic add of
+// the wavefront's total to *ctr (every lane must call it): the base of the
+// lane's entries in the list that *ctr counts
+__device__ inline int wave_reserve(int v, int32_t *ctr) {
+  const int lane = threadIdx.x % kWave;
+  int incl = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int u = __shfl_up(incl, off);
+    if (lane >= off) incl += u;
+  }
+  const int total = __shfl(incl, kWave - 1);
+  int base = 0;
+  if (lane == kWave - 1 && total > 0) base = atomicAdd(ctr, total);
+  base = __shfl(base, kWave - 1);
+  return base + incl - v;
+}
+
+// compare-exchange of (slot, position) pairs by slot
+__device__ inline void cswap(int &ka, int &va, int &kb, int &vb) {
+  if (kb < ka) {
+    const int k = ka, v = va;
+    ka = kb; va = vb; kb = k; vb = v;
+  }
+}
+
+// one thread per node.  Big nodes (more than kSmallNode members) are queued
+// for k_moments_chunks / k_moments_big (one atomic per wavefront).  Nodes of
+// at most kTinyNode members are summed in slot order, as new_tree meets
+// them: their member slots are sorted in registers by a sorting network.
+// Nodes of kTinyNode+1..kSmallNode members are summed in key order.
 __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= A.nnodes) return;
+  const bool in = node < A.nnodes;
   const bool is_root = node == 0;
-  const int n = is_root ? A.nact : A.count[node];
-  if (n > kSmallNode) {
-    // big node: its member range is reduced in chunks by k_moments_chunks
-    const int q = atomicAdd(A.big_count, 1);
-    const int nch = (n + kChunk - 1) / kChunk;
-    const int base = atomicAdd(A.big_count + 1, nch);
+  const int n = !in ? 0 : (is_root ? A.nact : A.count[node]);
+  const bool big = in && n > kSmallNode;
+  const int nch = big ? (n + kChunk - 1) / kChunk : 0;
+  const int q = wave_reserve(big ? 1 : 0, A.big_count);
+  const int base = wave_reserve(nch, A.big_count + 1);
+  if (!in) return;
+  if (big) {
     A.big_list[q] = (int)node;
     A.big_base[q] = base;
     for (int k = 0; k < nch; ++k) A.chunk_big[base + k] = q;
-    // its children follow it in preorder: node + 1, then each child's skip
-    const int end = A.meta[node].x;
-    for (int ch = (int)node + 1; ch < end; ch = A.meta[ch].x) {
-      const int cn = A.count[ch];
-      if (cn >= 2 && cn <= kSmallNode) A.sub_list[atomicAdd(A.big_count + 2, 1)] = ch;
-    }
-    return;
-  }
-  if (n >= 2 || (is_root && n >= 1)) {
-    // summed in slot order by k_moments_subtree: the root itself is a
-    // subtree when it holds at most kSmallNode points
-    if (is_root) A.sub_list[atomicAdd(A.big_count + 2, 1)] = 0;
     return;
   }
   const int f = A.first[node];
@@ -502,75 +514,33 @@ __global__ void __launch_bounds__(kBlock) k_moments_small(MomArgs A) {
     A.rec[node] = r;
     return;
   }
-}
-
-// Slot-order moments of every node in a subtree of at most kSmallNode
-// members, one wavefront per subtree.  The subtree's members (a contiguous
-// range of sorted positions, in key order) are sorted by slot across the
-// lanes (bitonic sort on (slot, position)); lane r then holds the r-th
-// member in slot order and stages its position and ghost in LDS.  Each
-// lane takes one node of the subtree (its nodes are the preorder range
-// [root, skip)) and adds, in slot order, the members whose sorted position
-// lies in the node's range -- the members of its list in new_tree, in the
-// order new_tree meets them.
-constexpr int kSubWaves = kBlock / kWave;
-
-__global__ void __launch_bounds__(kBlock) k_moments_subtree(MomArgs A) {
-  __shared__ int32_t sq[kSubWaves][kWave];
-  __shared__ double sx[kSubWaves][6][kWave];
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  const int nsub = A.big_count[2];
-  for (int w = blockIdx.x * kSubWaves + wid; w < nsub; w += gridDim.x * kSubWaves) {
-    const int f = A.sub_list[w];
-    const int first = f == 0 ? 0 : A.first[f];
-    const int n = f == 0 ? A.nact : A.count[f];  // 1..kSmallNode
-    // members by slot: key = slot (lanes past n sort last), value = the
-    // member's index in the subtree's key-order range
-    int key = lane < n ? A.slots[first + lane] : INT32_MAX;
-    int val = lane;
-    for (int k = 2; k <= kWave; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int pk = __shfl_xor(key, j), pv = __shfl_xor(val, j);
-        const bool up = (lane & k) == 0, low = (lane & j) == 0;
-        // the lower lane of a pair keeps the smaller key in an ascending
-        // run, the larger in a descending one
-        const bool take = (low == up) ? (pk < key) : (pk > key);
-        if (take) { key = pk; val = pv; }
-      }
-    }
-    if (lane < n) {
-      const int t = first + val;
-      sq[wid][lane] = val;
-      sx[wid][0][lane] = A.xs[t]; sx[wid][1][lane] = A.ys[t]; sx[wid][2][lane] = A.zs[t];
-      sx[wid][3][lane] = A.gxs[t]; sx[wid][4][lane] = A.gys[t]; sx[wid][5][lane] = A.gzs[t];
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int end = A.meta[f].x;
-    for (int base = f; base < end; base += kWave) {
-      const int v = base + lane;
-      const bool own = v < end;
-      const bool root = v == 0;
-      const int cnt = !own ? 0 : (root ? A.nact : A.count[v]);
-      // leaves are written by k_moments_small
-      const bool sum = own && (cnt >= 2 || (root && cnt >= 1));
-      const int lo = sum ? (root ? 0 : A.first[v] - first) : 0;
-      const int hi = sum ? lo + cnt : 0;
-      double s[15];
-      for (int k = 0; k < 15; ++k) s[k] = 0.;
-      for (int r = 0; r < n; ++r) {
-        const int q = sq[wid][r];
-        if (q >= lo && q < hi) {
-          const double x[3] = {sx[wid][0][r], sx[wid][1][r], sx[wid][2][r]};
-          const double g[3] = {sx[wid][3][r], sx[wid][4][r], sx[wid][5][r]};
-          add_member(x, g, s);
-        }
-      }
-      if (sum) finalize_node(A, v, cnt, s, root);
-    }
-    // the next subtree overwrites this wavefront's LDS rows
-    __builtin_amdgcn_wave_barrier();
+  for (int k = 0; k < 15; ++k) s[k] = 0.;
+  if (n <= kTinyNode) {
+    // the members' slots sorted with their positions by Batcher's 19-
+    // comparator network for 8 inputs (positions past n sort last)
+    int k0 = A.slots[f], v0 = 0;
+    int k1 = 1 < n ? A.slots[f + 1] : INT32_MAX, v1 = 1;
+    int k2 = 2 < n ? A.slots[f + 2] : INT32_MAX, v2 = 2;
+    int k3 = 3 < n ? A.slots[f + 3] : INT32_MAX, v3 = 3;
+    int k4 = 4 < n ? A.slots[f + 4] : INT32_MAX, v4 = 4;
+    int k5 = 5 < n ? A.slots[f + 5] : INT32_MAX, v5 = 5;
+    int k6 = 6 < n ? A.slots[f + 6] : INT32_MAX, v6 = 6;
+    int k7 = 7 < n ? A.slots[f + 7] : INT32_MAX, v7 = 7;
+    cswap(k0, v0, k1, v1); cswap(k2, v2, k3, v3); cswap(k4, v4, k5, v5); cswap(k6, v6, k7, v7);
+    cswap(k0, v0, k2, v2); cswap(k1, v1, k3, v3); cswap(k4, v4, k6, v6); cswap(k5, v5, k7, v7);
+    cswap(k1, v1, k2, v2); cswap(k5, v5, k6, v6);
+    cswap(k0, v0, k4, v4); cswap(k1, v1, k5, v5); cswap(k2, v2, k6, v6); cswap(k3, v3, k7, v7);
+    cswap(k2, v2, k4, v4); cswap(k3, v3, k5, v5);
+    cswap(k1, v1, k2, v2); cswap(k3, v3, k4, v4); cswap(k5, v5, k6, v6);
+    const int order[8] = {v0, v1, v2, v3, v4, v5, v6, v7};
+#pragma unroll
+    for (int r = 0; r < kTinyNode; ++r)
+      if (r < n) add_point(A, f + order[r], s);
+  } else {
+#pragma unroll 4
+    for (int t = f; t < f + n; ++t) add_point(A, t, s);
   }
+  finalize_node(A, node, n, s, is_root);
 }
 
 // one workgroup per chunk of a big node's member range; partial sums per
@@ -898,13 +868,10 @@ int tree_build(Ctx &c) {
   MA.partial = c.partial;
   MA.big_count = c.counters + 2;
   MA.slots = c.slot_s;
-  MA.sub_list = c.sub_list;
   hipLaunchKernelGGL(k_moments_small, dim3(grid_of(nnodes)), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
-  // one wavefront per subtree of >= 2 points: at most nact / 2 of them,
-  // about nact / 20 on random loops; waves past the list end return at once
-  hipLaunchKernelGGL(k_moments_subtree, dim3(nact / 64 + 1), dim3(kBlock), 0, st, MA);
-  QV_TRY(hipGetLastError());
+  // nodes of more than kSmallNode members: chunk partials, then one thread
+  // per node combines them in chunk order
   hipLaunchKernelGGL(k_moments_chunks, dim3(2048), dim3(kBlock), 0, st, MA);
   QV_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_moments_big, dim3(64), dim3(kBlock), 0, st, MA);

@@ -2,7 +2,7 @@
 and against the oracle, through the C ABI.
 
 Bars (FP64 path): tree occupancy, cell bounds, leaf indices and the eval
-counter identical; the moments of every node of at most 64 points
+counter identical; the moments of every node of at most 8 points
 identical; direct (BS) and tree velocities within 1e-12 relative per
 point; AB trajectories within 1e-15 absolute.  FP32C
 path: within 1e-5 relative per point of the FP64 reference.
@@ -68,9 +68,9 @@ def _check_tree(t, ref, L):
     assert t.shape == ref.shape
     # level, pcount, leaf index, corner and width: bit-identical
     assert np.array_equal(t[:, 0:7], ref[:, 0:7])
-    # moments of every node of at most 64 points (summed in slot order, as
+    # moments of every node of at most 8 points (summed in slot order, as
     # new_tree sums them): bit-identical
-    small = (ref[:, 1] <= 64) & (ref[:, 0] > 0)
+    small = (ref[:, 1] <= 8) & (ref[:, 0] > 0)
     assert np.array_equal(t[small, 7:22], ref[small, 7:22])
     _say("nodes", len(ref), "bit-identical", int(small.sum()))
     # bigger nodes (summed in chunks): equal up to summation order
