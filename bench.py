@@ -116,36 +116,56 @@ def error_vs_direct(fil, theta: float, u32, u64, n_sample: int = 200) -> dict:
     return out
 
 
-FLOP_PEAK_TFS = {"fp32c": 157.3, "fp64": 78.6}  # MI355X published fp32 / fp64
+FLOP_PEAK_TFS = {"fp32c": 157.3, "fp64": 78.6}  # MI355X published fp32 / fp64 vector
+UNPACKED_PEAK_TFS = 78.6  # one FMA per lane and cycle, FP32 or FP64 (no packed FP32)
 
 
-def roofline(res, precision: str, n: int, traffic):
-    """Walk kernel against the compute roofline (vector ALU; the walk has no
-    dense contraction, so the MFMA peaks of the same precision are the
-    published ceiling).  Algorithmic flops, counting +,-,*,/,sqrt as one:
-    12 per node test (per target and visited node), 44 per leaf segment,
-    64 per accepted cell, with the counts taken from the kernel's own
-    64-bit counters.  Also reported: the HBM view of the same launch with
-    the per-target node bytes (48 B per target; per visit 5 values, per cell
-    15, per leaf 6, at 4 or 8 bytes), which exceeds the HBM peak because a
-    wavefront's scalar node loads are shared by its 64 targets."""
+def walk_traffic(precision: str):
+    """HBM bytes per launch of the walk from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a plain copy's), with the
+    summary's file, command and workload: the counters cannot be read inside
+    this run."""
+    path = os.path.join(ROOT, "profiles", "walk_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        d = json.load(fh).get(f"k_walk_{precision}")
+    if not d:
+        return None, None
+    return d["bytes_per_launch"], {k: d[k] for k in ("source", "command", "workload")}
+
+
+def roofline(res, precision: str, n: int):
+    """The walk kernel against the vector-ALU roofline: the walk issues no
+    MFMA (no dense contraction), its arithmetic runs on the vector ALUs.
+    Algorithmic flops, counting +,-,*,/,sqrt as one: 12 per node test (per
+    target and visited node), 44 per leaf segment, 64 per accepted cell,
+    with the counts taken from the kernel's own 64-bit counters.  `peak` is
+    MI355X's published vector peak of the precision; FP32's 157.3 TFLOP/s
+    counts packed FP32 (two per lane and cycle), which this build does not
+    generate, so `frac_unpacked` states the fraction of the 78.6 TFLOP/s that
+    non-packed FP32 and FP64 reach.  `per_lane_bytes` counts the node data
+    each target's walk reads (48 B per target; per visit 5 values, per cell
+    15, per leaf 6, at 4 or 8 bytes): not HBM traffic -- one scalar load of
+    a node serves the 64 targets of a wavefront -- which `traffic` gives."""
     t = res["timings"]
     visits, leaves, cells = t["visits"], t["leaf_evals"], t["cell_evals"]
     flops = 12 * visits + 44 * leaves + 64 * cells
     tflops = flops / (res["walk_ms"] * 1e-3) / 1e12
     peak = FLOP_PEAK_TFS[precision]
     b = 8 if precision == "fp64" else 4
-    bytes_alg = n * 48 + b * (5 * visits + 15 * cells + 6 * leaves)
-    return {"bound": "mfma", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(tflops / peak, 4), "traffic": traffic,
+    bytes_lane = n * 48 + b * (5 * visits + 15 * cells + 6 * leaves)
+    traffic, source = walk_traffic(precision)
+    return {"bound": "valu", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(tflops / peak, 4),
+            "frac_unpacked": round(tflops / UNPACKED_PEAK_TFS, 4),
+            "traffic": traffic, "traffic_source": source,
             "kernel": f"k_walk<{precision}>", "flops_per_launch": int(flops),
             "walk_ms": round(res["walk_ms"], 3),
             "lane_utilisation": round(visits / (64.0 * max(1, t["wave_steps"])), 4),
             "counts": {"visits": visits, "leaves": leaves, "cells": cells,
                        "wave_steps": t["wave_steps"]},
-            "hbm_view": {"bytes_alg_per_launch": int(bytes_alg),
-                         "GB_per_s": round(bytes_alg / (res["walk_ms"] * 1e-3) / 1e9, 1),
-                         "peak": HBM_PEAK_GBS}}
+            "per_lane_bytes": int(bytes_lane)}
 
 
 PAIR_FLOPS = 52  # one FP32 segment term, counting FMA as 2: see run_ensemble
@@ -163,7 +183,7 @@ def run_direct(n_points: int = 262144, steps: int = 3) -> dict:
     fil = G.random_loops(0.1, 5e-4, target_points=n_points, seed=7, spacing=0.8, wrap=False)
     n = int(np.count_nonzero(fil.infront))
     pairs = n * (n - 2)  # every active source but the target and its behind
-    out = {"workload": "random loops, open box L=0.1 cm, delta=5e-4 cm, velocity BS",
+    out = {"workload": f"random loops, open box L=0.1 cm, delta=5e-4 cm, velocity BS, {n} points",
            "points": n, "pairs": pairs, "pair_flops": DIRECT_PAIR_FLOPS}
     for prec in ("fp64", "fp32c"):
         p = gpu.params(velocity="BS", tree_theta=0.0, boundary="open", box_size=0.1, delta=5e-4,
@@ -224,6 +244,82 @@ def run_ensemble(steps: int, seed: int = 20261202):
             "graph": True, "variant": chosen, "targets_per_unit": info["tile"],
             "work_units": info["nunits"], "tune_ms": [round(t, 4) for t in tune_ms],
             "error_vs_fp64": {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e ** 2)))}}
+
+
+def run_config3(fil, theta: float, steps: int) -> dict:
+    """The 1M-point tangle run as the reference's main loop runs it
+    (run.f90:36-169): `steps` steps of dt = 1e-6 s, FP32 far field, the host
+    topology step with the neighbour search every step, the boundary step;
+    wall clock from Python over the whole run.  Beside it, the same number of
+    steps without the line operations (qv_step: velocity, AB, wrap on the
+    device)."""
+    from qvort_amd import gpu, line
+    p = gpu.params(velocity="Tree", tree_theta=theta, boundary="periodic", box_size=0.2,
+                   delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6, precision="fp32c")
+    st = line.LineState.from_arrays(capacity=2 * fil.pcount + 64, x=fil.x,
+                                    infront=fil.infront, behind=fil.behind)
+    cnt = line.new_counts()
+    with gpu.Context(p) as c:
+        c.upload_filaments(fil)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c.velocity_only()
+            c.ab_step()
+            line.topology_step(c, p, st, cnt, search=True)
+            c.enforce_periodic()
+        wall = time.perf_counter() - t0
+    with gpu.Context(p) as c:
+        c.upload_filaments(fil)
+        c.step(1)
+        t0 = time.perf_counter()
+        c.step(steps)
+        wall_dev = time.perf_counter() - t0
+    return {"steps": steps, "dt": 1e-6, "wall_s": round(wall, 3),
+            "ms_per_step": round(wall / steps * 1e3, 3),
+            "counts": line.counts_dict(cnt),
+            "points_after": int(np.count_nonzero(st.infront)),
+            "without_line_ops": {"wall_s": round(wall_dev, 3),
+                                 "ms_per_step": round(wall_dev / steps * 1e3, 3)}}
+
+
+def run_adjoint(n_points: int = 8192, nsteps: int = 200) -> dict:
+    """The adjoint configuration: a periodic Kelvin-wave line of 8192 points
+    (L = 0.1 cm); one VJP of the FP64 direct-sum velocity (qv_vjp_device)
+    timed on the device stream, and the gradient of the line length after
+    200 AB steps checkpointed every 20 (forward, recomputation and backward
+    through the hand-written VJP), wall clock."""
+    import torch
+
+    from qvort_amd import adjoint as AD
+    from qvort_amd import geometry as G
+    fil = G.kelvin_line(n_points, 0.1, amp_sigma=2e-4, seed=20261202)
+    eng = AD.Engine(fil, box_size=0.1, boundary="periodic")
+    try:
+        x = torch.as_tensor(fil.x, device="cuda")
+        w = torch.ones_like(x)
+        out = torch.empty_like(x)
+        eng._bind()
+        eng.ctx.vjp_device(x.data_ptr(), w.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            eng.ctx.vjp_device(x.data_ptr(), w.data_ptr(), out.data_ptr())
+        e1.record()
+        torch.cuda.synchronize()
+        vjp_ms = e0.elapsed_time(e1) / 5
+        dt = 0.25 * AD.dt_max(0.1 / n_points)
+        t0 = time.perf_counter()
+        J, g = AD.gradient("length", fil.x, eng, nsteps, dt, checkpoint_every=20)
+        grad_s = time.perf_counter() - t0
+    finally:
+        eng.close()
+    n = n_points
+    pairs = 27 * n * (n - 2)
+    return {"workload": f"periodic Kelvin-wave line, {n} points, L=0.1 cm, 27 shells",
+            "vjp_ms": round(vjp_ms, 3), "vjp_pairs": pairs,
+            "gradient_steps": nsteps, "checkpoint_every": 20,
+            "gradient_wall_s": round(grad_s, 3)}
 
 
 def copy_calibration(gib: float = 2.0, reps: int = 5) -> None:
@@ -301,8 +397,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-next-rows", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "fp32c"], default=None,
-                    help="profile one precision (no JSON line)")
+    ap.add_argument("--config3-steps", type=int, default=100)
+    ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32c", "direct", "ensemble", "adjoint"],
+                    default=None, help="run one part for a profiler (no JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain copy of known size (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -315,6 +414,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "direct":
+        print(json.dumps(run_direct(steps=args.steps)), file=sys.stderr, flush=True)
+        return
+    if args.only == "ensemble":
+        print(json.dumps(run_ensemble(args.ensemble_steps)), file=sys.stderr, flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps(run_adjoint()), file=sys.stderr, flush=True)
+        return
     fil = workload(args.points)
     n = fil.n_active
     print(f"workload: {n} points, {len(fil.loop_len)} loops", file=sys.stderr, flush=True)
@@ -335,12 +443,6 @@ def main():
            "rms": float(np.sqrt(np.mean(e ** 2))),
            "max_over_rms_velocity": float(np.max(np.linalg.norm(
                r32["u"][act] - r64["u"][act], axis=1)) / np.sqrt(np.mean(nr ** 2)))}
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "walk_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("k_walk_fp32c_bytes_per_launch")
 
     ms32 = r32["wall_s"] / args.steps * 1e3
     ms64 = r64["wall_s"] / args.steps * 1e3
@@ -372,12 +474,20 @@ def main():
         "error": err,
         "evals_per_point": r64["timings"]["evals"] / n,
         "tree_nodes": r64["timings"]["nodes"],
-        "roofline": roofline(r32, "fp32c", n, traffic),
-        "roofline_fp64": roofline(r64, "fp64", n, None),
+        "roofline": roofline(r32, "fp32c", n),
+        "roofline_fp64": roofline(r64, "fp64", n),
     }
     if not args.no_direct:
         out["direct_sum"] = run_direct()
-        print(f"direct sum: {out['direct_sum']}", file=sys.stderr, flush=True)
+        out["direct_sum_65k"] = run_direct(65536)
+        print(f"direct sum: {out['direct_sum']} {out['direct_sum_65k']}", file=sys.stderr,
+              flush=True)
+    if not args.no_config3:
+        out["config3_run"] = run_config3(fil, args.theta, args.config3_steps)
+        print(f"config 3 run: {out['config3_run']}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint()
+        print(f"adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.ensemble_steps)
         print(f"ensemble: {out['ensemble']['ms_per_step']:.3f} ms/step", file=sys.stderr,
