@@ -470,7 +470,7 @@ int qv_destroy(qv_ctx *ctx) {
                 c->big_buf, c->partial, c->adj,
                 c->tree.rec, c->tree.meta, c->tree.mp, c->tree.mp32, c->tree.geo,
                 c->tree.first, c->tree.count, c->tree.level, c->tree.digit, c->tree.corner,
-                c->tree.linf, c->close_i, c->close_d, c->close_order, c->probe_buf, c->probe_part,
+                c->tree.linf, c->tree.tplain, c->tree.tclose, c->close_i, c->close_d, c->close_order, c->probe_buf, c->probe_part,
                 c->aux_tmp};
   for (void *p : ps) (void)hipFree(p);
   if (c->h_counters) (void)hipHostFree(c->h_counters);

@@ -39,6 +39,7 @@ struct CloseArgs {
   const NodeRec *rec;
   const double2 *geo;    // width
   const int32_t *count;  // points per node (root: all slots)
+  const double *tclose;  // smallest d2 with width/dist < 0.5 (-1: exact test)
   const int32_t *linf;
   const double *X, *Y, *Z;
   const int32_t *inf, *beh;
@@ -84,19 +85,28 @@ __global__ void __launch_bounds__(kBlock) k_closest_tree(CloseArgs A) {
     const int skip = uni(r.skip);
     const int32_t cnt = uni(A.count[n]);
     const double w = A.geo[n].x;
+    const double T = A.tclose[n];
     const bool mine = __builtin_amdgcn_inverse_ballot_w64(M);
     const double d0 = x0 - cx, d1 = x1 - cy, d2 = x2 - cz;
-    const double dist = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-    const bool taken = mine && (cnt == 1 || w / dist < .5);
-    if (taken && dist < bd) {
-      if (cnt > 1) {
-        best = 0;
-      } else {
-        const int32_t q = A.linf[n];
-        const int32_t j = q >= 0 ? A.beh[q] : 0;  // q < 0 only for a 1-slot root
-        if (j != i + 1 && fi != j && bi != j) {
-          best = j;
-          bd = dist;
+    const double dsq = (d0 * d0 + d1 * d1) + d2 * d2;
+    // width/sqrt(dsq) < 0.5 is monotone in dsq (a correctly rounded square
+    // root and division): dsq >= T decides it where T >= 0
+    const bool open_test = T >= 0. ? dsq >= T : w / sqrt(dsq) < .5;
+    const bool taken = mine && (cnt == 1 || open_test);
+    // sqrt(dsq) < bd implies dsq < bd^2, and bd * bd * (1 + 1e-7) exceeds bd^2
+    // despite its two roundings: lanes failing this cannot be closer
+    if (taken && dsq < (bd * bd) * 1.0000001) {
+      const double dist = sqrt(dsq);
+      if (dist < bd) {
+        if (cnt > 1) {
+          best = 0;
+        } else {
+          const int32_t q = A.linf[n];
+          const int32_t j = q >= 0 ? A.beh[q] : 0;  // q < 0 only for a 1-slot root
+          if (j != i + 1 && fi != j && bi != j) {
+            best = j;
+            bd = dist;
+          }
         }
       }
     }
@@ -182,6 +192,7 @@ hipError_t launch_closest(Ctx &c, bool tree) {
   A.rec = c.tree.rec;
   A.geo = c.tree.geo;
   A.count = c.tree.count;
+  A.tclose = c.tree.tclose;
   A.linf = c.tree.linf;
   A.X = c.x.x; A.Y = c.x.y; A.Z = c.x.z;
   A.inf = c.infront;
@@ -189,6 +200,10 @@ hipError_t launch_closest(Ctx &c, bool tree) {
   A.cl = c.close_i;
   A.cd = c.close_d;
   const dim3 g((c.pcount + kBlock - 1) / kBlock);
+  if (tree && !c.tree.close_valid) {
+    const hipError_t e = launch_close_thresholds(c);
+    if (e != hipSuccess) return e;
+  }
   if (tree)
     hipLaunchKernelGGL(k_closest_tree, g, dim3(kBlock), 0, c.stream, A);
   else

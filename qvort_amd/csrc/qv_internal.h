@@ -64,6 +64,10 @@ struct Tree {
   // build (plain_valid); -1: exact test at every visit
   double *tplain = nullptr;
   bool plain_valid = false;
+  // the same for the neighbour search's test width/dist < 0.5 (closest_tree);
+  // computed on the first search after a build (close_valid)
+  double *tclose = nullptr;
+  bool close_valid = false;
   // build-only arrays (per node)
   int32_t *first = nullptr;  // first sorted position
   int32_t *count = nullptr;  // member count (root: pcount slots)
@@ -210,6 +214,8 @@ hipError_t launch_hilbert_keys(hipStream_t st, int n, double L, const double *x,
                                int32_t *idx);
 // the probe walk's per-node thresholds (Tree::tplain) for the current tree
 hipError_t launch_plain_thresholds(Ctx &c);
+// the neighbour search's per-node thresholds (Tree::tclose)
+hipError_t launch_close_thresholds(Ctx &c);
 // slots 0..n-1 (positions pos) in Hilbert-curve order into order[0..n);
 // with inf, slots with infront = 0 go last.  Uses the build's key buffers
 // and the sort scratch (tmp, tmp_bytes; grown as needed) on stream st.

@@ -662,7 +662,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   int64_t cap = need + need / 4 + 1024;
   (void)hipFree(t.rec); (void)hipFree(t.meta); (void)hipFree(t.mp); (void)hipFree(t.mp32); (void)hipFree(t.geo);
   (void)hipFree(t.first); (void)hipFree(t.count); (void)hipFree(t.level); (void)hipFree(t.digit); (void)hipFree(t.corner);
-  (void)hipFree(t.linf); (void)hipFree(t.tplain);
+  (void)hipFree(t.linf); (void)hipFree(t.tplain); (void)hipFree(t.tclose);
   t = Tree{};
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&t.rec, cap * sizeof(NodeRec)))) return e;
@@ -677,6 +677,7 @@ hipError_t grow_tree(Tree &t, int64_t need) {
   if ((e = hipMalloc(&t.corner, cap * 3 * sizeof(double)))) return e;
   if ((e = hipMalloc(&t.linf, cap * sizeof(int32_t)))) return e;
   if ((e = hipMalloc(&t.tplain, cap * sizeof(double)))) return e;
+  if ((e = hipMalloc(&t.tclose, cap * sizeof(double)))) return e;
   t.cap = cap;
   return hipSuccess;
 }
@@ -706,6 +707,19 @@ hipError_t launch_plain_thresholds(Ctx &c) {
                      c.tree.tplain);
   const hipError_t e = hipGetLastError();
   if (e == hipSuccess) c.tree.plain_valid = true;
+  return e;
+}
+
+hipError_t launch_close_thresholds(Ctx &c) {
+  const int64_t nn = c.tree.nnodes;
+  if (nn <= 0) return hipSuccess;
+  // closest_tree takes a many-point node when width/dist < 0.5
+  // (tree.f90:247-249): the plain test with theta = 0.5
+  hipLaunchKernelGGL(k_plain_thresholds, dim3((unsigned)((nn + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, c.stream, nn, c.tree.geo, c.tree.count, 0.5,
+                     c.tree.tclose);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) c.tree.close_valid = true;
   return e;
 }
 
@@ -817,6 +831,7 @@ int tree_build(Ctx &c) {
   QV_TRY(grow_tree(c.tree, nnodes));
   c.tree.nnodes = nnodes;
   c.tree.plain_valid = false;
+  c.tree.close_valid = false;
   // 6. node records
   NodeArgs NA;
   NA.nact = nact;
