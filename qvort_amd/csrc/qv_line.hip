@@ -9,8 +9,20 @@
 // operation order (no contraction), so that the state after each operation
 // is the reference's bit for bit.  Slot indices are 0-based here, links
 // 1-based as in f(:).
+//
+// What a slot's test reads before the loop reaches it is computed first, in
+// parallel (pinsert: each segment's length; premove: whether the point two
+// ahead is within 0.499 (delta_i + delta_j)), and the sequential loop takes
+// that value unless an earlier slot's operation changed one of its inputs
+// (the slot's position, ghost or link, or those of the slots it reads),
+// which it marks in a flag per slot; then it evaluates the test again, with
+// the same expression.  The validation of the links is parallel too.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "qv_internal.h"
 #include "qv_lia.h"
@@ -21,11 +33,35 @@
 namespace qv {
 namespace {
 
+// f(lo, hi) over [0, n) on up to 16 threads (one for small n)
+template <class F>
+void par_for(int n, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::min(16u, hw > 0 ? hw : 1u);
+  if (n < (1 << 16) || nt <= 1) {
+    f(0, n);
+    return;
+  }
+  const int chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) {
+    const int lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo < hi) th.emplace_back(f, lo, hi);
+  }
+  f(0, std::min(n, chunk));
+  for (auto &x : th) x.join();
+}
+
+// per-thread scratch of the pre-pass, kept between calls
+thread_local std::vector<double> tl_pre;
+thread_local std::vector<uint8_t> tl_flag, tl_dirty;
+
 struct Line {
   const qv_params &p;
   qv_line_state &s;
   qv_line_counts &c;
   bool per[3];
+  uint8_t *dirty = nullptr;  // slots whose inputs changed in this call
 
   Line(const qv_params &p_, qv_line_state &s_, qv_line_counts &c_) : p(p_), s(s_), c(c_) {
     per[0] = p.boundary == QV_BC_PERIODIC;
@@ -45,6 +81,7 @@ struct Line {
   // moved by one box length where it lies more than half a box away from
   // x(i) along a periodic axis
   void ghost_p(int i) {
+    if (dirty) dirty[i] = 1;
     const int a = inf(i), b = beh(i), aa = inf(a), bb = beh(b);
     const double *x = X(i);
     const int src[4] = {a, b, aa, bb};
@@ -76,6 +113,7 @@ struct Line {
 
   // clear_particle (general.f90:11-22); u3 is left as it is
   void clear(int i) {
+    if (dirty) dirty[i] = 1;
     for (double *v : {X(i), U(i), s.u1 + 3 * (size_t)i, s.u2 + 3 * (size_t)i, GI(i), GB(i),
                       GII(i), GBB(i)})
       v[0] = v[1] = v[2] = 0.;
@@ -122,13 +160,23 @@ struct Line {
     for (int i = 0; i < s.pcount; ++i) nact += s.infront[i] > 0;
     if (nact == 0) return QV_ERR_LINE_EMPTY;
     const int old = s.pcount;
+    // each active slot's segment length (the test's input), in parallel
+    std::vector<double> &pre = tl_pre;
+    pre.resize(old);
+    par_for(old, [&](int lo, int hi) {
+      for (int i = lo; i < hi; ++i)
+        if (s.infront[i] != 0) pre[i] = lia_dist(X(i), GI(i));
+    });
+    // written by ghost_p (the ghosts, and the new point's position)
+    tl_dirty.assign((size_t)s.capacity, 0);
+    dirty = tl_dirty.data();
     double total = 0.;
     // the first empty slot (minloc of infront): empty slots are only filled
     // here, never made, so it only moves forward and one cursor finds it
     int hole = 0;
     for (int i = 0; i < old; ++i) {
       if (s.infront[i] == 0) continue;
-      const double disti = lia_dist(X(i), GI(i));
+      const double disti = dirty[i] ? lia_dist(X(i), GI(i)) : pre[i];
       total = total + disti;
       // the point's delta factor is 1 (delta_adapt F)
       if (!(disti > 1. * p.delta)) continue;
@@ -218,15 +266,33 @@ struct Line {
     return QV_OK;
   }
 
+  // premove's test for slot i with the current links (both points' delta
+  // factors are 1: delta_adapt F)
+  bool premove_test(int i) {
+    const int aa = inf(inf(i));
+    const double distii = distf(i, aa);
+    return distii < 0.499 * p.delta * (1. + 1.);
+  }
+
   int premove() {
-    for (int i = 0; i < s.pcount; ++i) {
+    const int n = s.pcount;
+    std::vector<uint8_t> &hit = tl_flag;
+    hit.resize(n);
+    par_for(n, [&](int lo, int hi) {
+      for (int i = lo; i < hi; ++i) hit[i] = s.infront[i] != 0 && premove_test(i);
+    });
+    // the test reads infront(i), infront(a), x(i) and x(aa): a change of
+    // any of them marks its slot (the link set below, clear)
+    tl_dirty.assign((size_t)n, 0);
+    dirty = tl_dirty.data();
+    for (int i = 0; i < n; ++i) {
       if (s.infront[i] == 0) continue;
       const int a = inf(i), aa = inf(a);
-      const double distii = distf(i, aa);
-      // both points' delta factors are 1 (delta_adapt F)
-      if (!(distii < 0.499 * p.delta * (1. + 1.))) continue;
+      const bool close = (dirty[i] || dirty[a] || dirty[aa]) ? premove_test(i) : hit[i] != 0;
+      if (!close) continue;
       s.behind[aa] = i + 1;
       s.infront[i] = aa + 1;
+      dirty[i] = 1;
       clear(a);
       loop_killer(i);
       c.remove_count += 1;
@@ -306,14 +372,20 @@ int check(const qv_params *p, qv_line_state *s, qv_line_counts *c) {
   if (p->deriv_order != 2 && p->deriv_order != 4) return QV_ERR_ARG;
   if (p->boundary < QV_BC_OPEN || p->boundary > QV_BC_OPENXY || !(p->box_size > 0.))
     return QV_ERR_ARG;
-  for (int i = 0; i < s->pcount; ++i) {
-    const int a = s->infront[i], b = s->behind[i];
-    if (a < 0 || a > s->pcount || b < 0 || b > s->pcount) return QV_ERR_ARG;
-    if ((a == 0) != (b == 0)) return QV_ERR_ARG;
-    if (a != 0 && (s->infront[a - 1] == 0 || s->behind[b - 1] == 0)) return QV_ERR_ARG;
-    if (s->closest[i] < 0 || s->closest[i] > s->pcount) return QV_ERR_ARG;
-  }
-  return QV_OK;
+  std::atomic<bool> bad{false};
+  const int n = s->pcount;
+  par_for(n, [&](int lo, int hi) {
+    bool ok = true;
+    for (int i = lo; i < hi && ok; ++i) {
+      const int a = s->infront[i], b = s->behind[i];
+      if (a < 0 || a > n || b < 0 || b > n) ok = false;
+      else if ((a == 0) != (b == 0)) ok = false;
+      else if (a != 0 && (s->infront[a - 1] == 0 || s->behind[b - 1] == 0)) ok = false;
+      else if (s->closest[i] < 0 || s->closest[i] > n) ok = false;
+    }
+    if (!ok) bad.store(true, std::memory_order_relaxed);
+  });
+  return bad.load() ? QV_ERR_ARG : QV_OK;
 }
 
 }  // namespace
