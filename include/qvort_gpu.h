@@ -123,8 +123,12 @@ int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x,
  * that held a point and is empty now is cleared as clear_particle clears
  * it (general.f90:11-22).  Call it right after pinsert (the new points'
  * velocities are read off the links pinsert left) and after precon /
- * premove.  The copies of a topology step are then x down, the ghosts down
- * and x and the links up twice, instead of the whole state both ways. */
+ * premove.  x may be NULL when no slot becomes a point (after precon and
+ * premove, which only relink and clear): the device keeps its positions and
+ * zeroes the cleared slots' x; a slot that becomes a point then returns
+ * QV_ERR_ARG, with the links already applied.  The copies of a topology step
+ * are then x down, the ghosts down, x and the links up after pinsert, and
+ * the links up after premove, instead of the whole state both ways. */
 int qv_upload_topology(qv_ctx *ctx, int32_t pcount, const double *x,
                        const int32_t *infront, const int32_t *behind);
 /* AB history f(:)%u1..u3 (NULL = leave as is) */

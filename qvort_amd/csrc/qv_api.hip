@@ -3,6 +3,7 @@
 // reference routine each entry point replaces: include/qvort_gpu.h.
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -537,20 +538,30 @@ int qv_upload(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infro
 
 int qv_upload_topology(qv_ctx *ctx, int32_t pcount, const double *x, const int32_t *infront,
                        const int32_t *behind) {
-  if (!ctx || !x || !infront || !behind) return QV_ERR_ARG;
+  if (!ctx || !infront || !behind) return QV_ERR_ARG;
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (c->pcount == 0) return QV_ERR_STATE;
   if (pcount < c->pcount) return QV_ERR_ARG;  // slots are never given back
-  int32_t nact = 0;
-  bool consistent = true;
-  for (int32_t i = 0; i < pcount; ++i) {
-    const int32_t a = infront[i], b = behind[i];
-    if (a == 0) continue;
-    if (a < 1 || a > pcount || b < 1 || b > pcount) return QV_ERR_ARG;
-    if (infront[a - 1] == 0 || infront[b - 1] == 0) return QV_ERR_ARG;
-    if (behind[a - 1] != i + 1 || infront[b - 1] != i + 1) consistent = false;
-    ++nact;
-  }
+  std::atomic<int32_t> nact_sum{0};
+  std::atomic<bool> bad{false}, inconsistent{false};
+  par_for(pcount, [&](int lo, int hi) {
+    int32_t k = 0;
+    bool ok = true, cons = true;
+    for (int32_t i = lo; i < hi && ok; ++i) {
+      const int32_t a = infront[i], b = behind[i];
+      if (a == 0) continue;
+      if (a < 1 || a > pcount || b < 1 || b > pcount) ok = false;
+      else if (infront[a - 1] == 0 || infront[b - 1] == 0) ok = false;
+      else if (behind[a - 1] != i + 1 || infront[b - 1] != i + 1) cons = false;
+      ++k;
+    }
+    nact_sum += k;
+    if (!ok) bad.store(true);
+    if (!cons) inconsistent.store(true);
+  });
+  if (bad.load()) return QV_ERR_ARG;
+  const int32_t nact = nact_sum.load();
+  const bool consistent = !inconsistent.load();
   if (hipSetDevice(c->device) != hipSuccess) return QV_ERR_HIP;
   const int32_t n_old = c->pcount;
   int rc = reserve(*c, pcount, true);
@@ -562,14 +573,23 @@ int qv_upload_topology(qv_ctx *ctx, int32_t pcount, const double *x, const int32
                      c->stream));
   TRY(hipMemcpyAsync(c->slot_s, behind, pcount * sizeof(int32_t), hipMemcpyHostToDevice,
                      c->stream));
-  TRY(launch_topology_apply(*c, pcount, n_old, c->slot, c->slot_s, c->nnode_of));
+  // counters[7]: a slot became a point while the positions were kept
+  int32_t *flag = c->counters + 7;
+  if (!x) TRY(hipMemsetAsync(flag, 0, sizeof(int32_t), c->stream));
+  TRY(launch_topology_apply(*c, pcount, n_old, c->slot, c->slot_s, c->nnode_of, !x, flag));
   TRY(hipMemcpyAsync(c->infront, c->slot, pcount * sizeof(int32_t), hipMemcpyDeviceToDevice,
                      c->stream));
   TRY(hipMemcpyAsync(c->behind, c->slot_s, pcount * sizeof(int32_t), hipMemcpyDeviceToDevice,
                      c->stream));
-  rc = upload_vec(*c, x, c->x);
-  if (rc) return rc;
+  if (x) {
+    rc = upload_vec(*c, x, c->x);
+    if (rc) return rc;
+  } else {
+    TRY(hipMemcpyAsync(c->h_counters + 7, flag, sizeof(int32_t), hipMemcpyDeviceToHost,
+                       c->stream));
+  }
   TRY(hipStreamSynchronize(c->stream));
+  if (!x && c->h_counters[7] != 0) return QV_ERR_ARG;
   c->nact = nact;
   c->links_consistent = consistent;
   c->have_velocity = false;

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <thread>
+#include <vector>
+
 #include "../../include/qvort_gpu.h"
 
 namespace qv {
@@ -198,8 +202,11 @@ hipError_t launch_open_check(Ctx &c, int32_t *flag);
 // the per-slot rules of a topology upload: new points (pinsert) and cleared
 // slots, from the device's old links and the uploaded new ones (device
 // buffers); mark: n_new ints of scratch
+// keep_x: the positions were not uploaded; cleared slots get x = 0 and a
+// slot that becomes a point sets *flag = 1 (it has no position)
 hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *new_inf,
-                                 const int32_t *new_beh, int32_t *mark);
+                                 const int32_t *new_beh, int32_t *mark, bool keep_x,
+                                 int32_t *flag);
 // clear_particle's zeroes in the m slots of d_list (device)
 hipError_t launch_clear_slots(Ctx &c, int m, const int32_t *d_list);
 hipError_t launch_direct(Ctx &c);
@@ -237,5 +244,24 @@ hipError_t launch_closest(Ctx &c, bool tree);
 hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, double *d_g,
                         double *d_work,
                         bool tree, bool ref_scale, bool empty_core);
+
+// host: f(lo, hi) over [0, n) on up to 16 threads (one thread for small n)
+template <class F>
+void par_for(int n, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::min(16u, hw > 0 ? hw : 1u);
+  if (n < (1 << 16) || nt <= 1) {
+    f(0, n);
+    return;
+  }
+  const int chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) {
+    const int lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo < hi) th.emplace_back(f, lo, hi);
+  }
+  f(0, std::min(n, chunk));
+  for (auto &x : th) x.join();
+}
 
 }  // namespace qv

@@ -33,25 +33,6 @@
 namespace qv {
 namespace {
 
-// f(lo, hi) over [0, n) on up to 16 threads (one for small n)
-template <class F>
-void par_for(int n, F f) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  const int nt = (int)std::min(16u, hw > 0 ? hw : 1u);
-  if (n < (1 << 16) || nt <= 1) {
-    f(0, n);
-    return;
-  }
-  const int chunk = (n + nt - 1) / nt;
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) {
-    const int lo = t * chunk, hi = std::min(n, lo + chunk);
-    if (lo < hi) th.emplace_back(f, lo, hi);
-  }
-  f(0, std::min(n, chunk));
-  for (auto &x : th) x.join();
-}
-
 // per-thread scratch of the pre-pass, kept between calls
 thread_local std::vector<double> tl_pre;
 thread_local std::vector<uint8_t> tl_flag, tl_dirty;

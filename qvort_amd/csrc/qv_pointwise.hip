@@ -218,17 +218,23 @@ __global__ void k_clear_slots(int m, const int32_t *__restrict__ list, ClearArgs
 // too); a slot that held a point and is empty now was cleared by
 // clear_particle (general.f90:11-22): velocity, u1, u2, the LIA and
 // nonlocal parts and the ghosts are zeroed.  mark[s] = 1 for a new point.
+// With keep_x (positions not uploaded), a cleared slot's x is zeroed here
+// and a new point sets *flag.
 __global__ void k_topo_mark(int n_new, int n_old, const int32_t *__restrict__ old_inf,
                             const int32_t *__restrict__ new_inf, int32_t *__restrict__ mark,
                             ClearArgs A, double *__restrict__ u3x, double *__restrict__ u3y,
-                            double *__restrict__ u3z) {
+                            double *__restrict__ u3z, int keep_x, double *__restrict__ xx,
+                            double *__restrict__ xy, double *__restrict__ xz,
+                            int32_t *__restrict__ flag) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_new) return;
   const bool was = s < n_old && old_inf[s] != 0;
   const bool now = new_inf[s] != 0;
   mark[s] = (!was && now) ? 1 : 0;
+  if (keep_x && !was && now) atomicOr(flag, 1);
   if (was && !now) {
     for (int k = 0; k < ClearArgs::kArrays; ++k) A.a[k][s] = 0.;
+    if (keep_x) { xx[s] = 0.; xy[s] = 0.; xz[s] = 0.; }
   } else if (!was && now) {
     // u1, u2: entries 3..8 of the list (u, u1, u2, ...)
     for (int k = 3; k < 9; ++k) A.a[k][s] = 0.;
@@ -409,7 +415,8 @@ hipError_t launch_open_check(Ctx &c, int32_t *flag) {
 }
 
 hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *new_inf,
-                                 const int32_t *new_beh, int32_t *mark) {
+                                 const int32_t *new_beh, int32_t *mark, bool keep_x,
+                                 int32_t *flag) {
   if (n_new <= 0) return hipSuccess;
   ClearArgs A;
   Vec3 vs[] = {c.u, c.u1, c.u2, c.uli, c.ubs, c.gi, c.gb, c.gii, c.gbb};
@@ -420,7 +427,8 @@ hipError_t launch_topology_apply(Ctx &c, int n_new, int n_old, const int32_t *ne
     A.a[k++] = v.z;
   }
   hipLaunchKernelGGL(k_topo_mark, dim3(grid(n_new)), dim3(kBlock), 0, c.stream, n_new, n_old,
-                     c.infront, new_inf, mark, A, c.u3.x, c.u3.y, c.u3.z);
+                     c.infront, new_inf, mark, A, c.u3.x, c.u3.y, c.u3.z, keep_x ? 1 : 0,
+                     c.x.x, c.x.y, c.x.z, flag);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_topo_new_velocity, dim3(grid(n_new)), dim3(kBlock), 0, c.stream, n_new,

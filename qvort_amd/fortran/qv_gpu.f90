@@ -130,7 +130,7 @@ module qv_gpu
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: pcount
-      real(c_double), intent(in) :: x(3,*)
+      type(c_ptr), value :: x
       integer(c_int32_t), intent(in) :: infront(*), behind(*)
     end function
     !> the device links; they change on the device only when the boundary

@@ -747,13 +747,17 @@ contains
     ls = line_state()
     call qv_check(qv_line_pinsert(p, ls, counts), 'line.mod:pinsert')
     pcount = ls%pcount
-    call qv_check(qv_upload_topology(ctx, pcount, x, infront, behind), 'qv_upload_topology')
+    call qv_check(qv_upload_topology(ctx, pcount, c_loc(x), infront, behind), &
+                  'qv_upload_topology')
     if (mod(itime, recon_shots) == 0) then
       call qv_check(qv_closest(ctx, closest, closestd), 'qv_closest')
       ls = line_state()
       if (.not. switch_off_recon) call qv_check(qv_line_precon(p, ls, counts), 'line.mod:precon')
       call qv_check(qv_line_premove(p, ls, counts), 'line.mod:premove')
-      call qv_check(qv_upload_topology(ctx, pcount, x, infront, behind), 'qv_upload_topology')
+      ! precon and premove only relink and clear: the device keeps its
+      ! positions and zeroes the cleared slots'
+      call qv_check(qv_upload_topology(ctx, pcount, c_null_ptr, infront, behind), &
+                    'qv_upload_topology')
     end if
   end subroutine
 

@@ -230,14 +230,20 @@ class Context:
 
     def upload_topology(self, x, infront, behind) -> None:
         """Positions and links after host line operations, keeping the
-        device's velocities and AB history (qv_upload_topology)."""
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        device's velocities and AB history (qv_upload_topology).  x=None
+        keeps the device's positions (no slot may become a point; cleared
+        slots get x = 0 on the device)."""
         inf = np.ascontiguousarray(infront, dtype=np.int32)
         beh = np.ascontiguousarray(behind, dtype=np.int32)
         n = inf.shape[0]
-        if x.shape != (n, 3) or beh.shape != (n,):
-            raise ValueError("x must be (pcount,3), links (pcount,)")
-        _check(lib().qv_upload_topology(self._h, n, _dp(x), _ip(inf), _ip(beh)),
+        if x is not None:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            if x.shape != (n, 3):
+                raise ValueError("x must be (pcount,3)")
+        if beh.shape != (n,):
+            raise ValueError("links must be (pcount,)")
+        _check(lib().qv_upload_topology(self._h, n, None if x is None else _dp(x), _ip(inf),
+                                        _ip(beh)),
                "qv_upload_topology")
         self.pcount = n
 

@@ -129,7 +129,9 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
     the reference does after its ghostp.  The host owns the topology: `st`
     holds the current links (and closest/closestd) between calls; its
     velocity and history arrays are not kept up to date (the device's are
-    the state); in openx / openxy boxes the links come back first, since the
+    the state), nor are its positions after precon and premove (the device
+    zeroes the slots they clear); in openx / openxy boxes the links come
+    back first, since the
     device's boundary step changes them when a loop leaves through an open
     face."""
     n = ctx.pcount
@@ -157,4 +159,5 @@ def topology_step(ctx: gpu.Context, p: gpu.QvParams, st: LineState,
         ctx.closest(st.closest, st.closestd, inplace=True)
         precon(p, st, counts)
         premove(p, st, counts)
-        ctx.upload_topology(st.x, st.infront, st.behind)
+        # they only relink and clear slots: the device keeps its positions
+        ctx.upload_topology(None, st.infront, st.behind)

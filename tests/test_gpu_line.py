@@ -178,5 +178,16 @@ def test_topology_upload_keeps_device_history(name):
         ctx.closest(st.closest, st.closestd, inplace=True)
         line.precon(p, st, c)
         line.premove(p, st, c)
-        ctx.upload_topology(st.x, st.infront, st.behind)
+        # precon and premove only relink and clear: the device keeps its
+        # positions and zeroes the cleared slots', which must give the host's
+        ctx.upload_topology(None, st.infront, st.behind)
         check(ctx, "premove")
+        x, _, _, _, _ = ctx.download()
+        assert np.array_equal(x, st.x)
+        # a slot that becomes a point needs its position
+        if np.any(st.infront == 0):
+            inf2, beh2 = st.infront.copy(), st.behind.copy()
+            hole = int(np.flatnonzero(inf2 == 0)[0])
+            inf2[hole] = beh2[hole] = hole + 1  # a one-point loop
+            with pytest.raises(RuntimeError):
+                ctx.upload_topology(None, inf2, beh2)
