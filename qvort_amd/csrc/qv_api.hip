@@ -765,7 +765,11 @@ int qv_closest(qv_ctx *ctx, int32_t *closest, double *closestd) {
   }
   const size_t n = c->pcount;
   TRY(hipMemcpyAsync(c->close_i, closest, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  TRY(hipMemcpyAsync(c->close_d, closestd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // pclose_tree sets closestd = 100 for every slot first (tree.f90:224-225), and
+  // so does the tree search; pclose keeps the empty slots' values
+  if (!tree)
+    TRY(hipMemcpyAsync(c->close_d, closestd, n * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream));
   if (tree) {
     // the search's slot order; the slot buffers hold cap >= pcount entries
     int rc = join_aux(*c);
