@@ -301,7 +301,10 @@ int qv_batch_info(qv_batch *batch, int32_t *nvariants, int32_t *variant, int32_t
  * touches, exactly as the reference does.  Run.in options they implement:
  * recon_type 'original', delta_adapt F, phonon_emission F, recon_info F,
  * no mirror; the diagnostic logs (removed_curv.log, small_loops.log) are
- * not written. */
+ * not written.  The slot loops run in the reference's order; the per-slot
+ * tests they read, and the validation of the links, are computed first on
+ * up to 16 host threads (the environment variable QV_HOST_THREADS sets the
+ * number), which changes no result. */
 typedef struct qv_line_state {
   int32_t pcount;    /* slots in use (size(f)); pinsert may raise it */
   int32_t capacity;  /* slots allocated in every array below */

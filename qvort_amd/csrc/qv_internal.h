@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -245,11 +246,13 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
                         double *d_work,
                         bool tree, bool ref_scale, bool empty_core);
 
-// host: f(lo, hi) over [0, n) on up to 16 threads (one thread for small n)
+// host: f(lo, hi) over [0, n) on up to 16 threads, or QV_HOST_THREADS of
+// them when that is set (one thread for small n)
 template <class F>
 void par_for(int n, F f) {
   const unsigned hw = std::thread::hardware_concurrency();
-  const int nt = (int)std::min(16u, hw > 0 ? hw : 1u);
+  int nt = (int)std::min(16u, hw > 0 ? hw : 1u);
+  if (const char *e = std::getenv("QV_HOST_THREADS")) nt = std::max(1, std::min(64, std::atoi(e)));
   if (n < (1 << 16) || nt <= 1) {
     f(0, n);
     return;

@@ -132,3 +132,76 @@ def test_pinsert_capacity_and_empty_line_errors():
     with pytest.raises(gpu.QvortError) as ei:
         line.premove(p, bad, line.new_counts())
     assert ei.value.status == gpu.QV_ERR_ARG
+
+
+def _big_state(seed):
+    """A 120,000-point state of random loops (periodic cube L = 0.2 cm,
+    delta = 5e-4 cm) with ghosts from its links, every 97th segment
+    stretched past delta (pinsert inserts), every 89th point moved next to
+    the point two ahead (premove removes) and closest / closestd pointing
+    every 53rd point at a point moved next to it from elsewhere, closer
+    than delta / 2 (precon tests them): above the size from which the tests are computed
+    on several host threads."""
+    from qvort_amd import geometry as G
+    from qvort_amd import line
+    fil = G.random_loops(0.2, 5e-4, target_points=120_000, seed=seed, spacing=0.8)
+    n = fil.pcount
+    x = fil.x.copy()
+    inf, beh = fil.infront.copy(), fil.behind.copy()
+    act = np.flatnonzero(inf)
+    rng = np.random.default_rng(seed)
+    for i in act[::97]:
+        x[inf[i] - 1] += (x[inf[i] - 1] - x[i]) * 1.5
+    for i in act[::89]:
+        aa = inf[inf[i] - 1] - 1
+        x[i] = x[aa] + 1e-5
+    # every 53rd point gets a point of elsewhere moved next to it
+    pairs = []
+    for i in act[::53]:
+        j = int(act[rng.integers(act.size)])
+        if j in (i, inf[i] - 1, beh[i] - 1):
+            continue
+        x[j] = x[i] + np.array([0., 1e-4, 0.])
+        pairs.append((i, j))
+    L = 0.2
+
+    def ghost(src):
+        g = np.zeros_like(x)
+        g[act] = x[src[act] - 1]
+        d = x - g
+        g[d > L / 2] += L
+        g[d < -L / 2] -= L
+        return g
+    gi, gb = ghost(inf), ghost(beh)
+    closest = np.zeros(n, np.int32)
+    closestd = np.full(n, 100.)
+    for i, j in pairs:
+        closest[i] = j + 1
+        closestd[i] = 1e-4
+    return line.LineState.from_arrays(capacity=2 * n, pinned=False, x=x, infront=inf,
+                                      behind=beh, ghosti=gi, ghostb=gb, closest=closest,
+                                      closestd=closestd)
+
+
+def test_threaded_pre_pass_changes_nothing(monkeypatch):
+    """The line operations on a state past the threading threshold give the
+    same state and counters on one host thread and on eight."""
+    from qvort_amd import gpu, line
+    p = gpu.params(velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.2,
+                   delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6)
+    runs = []
+    for threads in ("1", "8"):
+        monkeypatch.setenv("QV_HOST_THREADS", threads)
+        st = _big_state(20261203)
+        c = line.new_counts()
+        line.pinsert(p, st, c)
+        line.precon(p, st, c)
+        line.premove(p, st, c)
+        runs.append((st, line.counts_dict(c)))
+    (a, ca), (b, cb) = runs
+    assert ca == cb
+    assert ca["inserted"] > 0 and ca["recon_count"] > 0 and ca["remove_count"] > 0
+    assert a.pcount == b.pcount
+    for k in VEC + ("infront", "behind", "closest", "closestd"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    _loops_ok(a)
