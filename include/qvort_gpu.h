@@ -304,7 +304,8 @@ int qv_batch_info(qv_batch *batch, int32_t *nvariants, int32_t *variant, int32_t
  * not written.  The slot loops run in the reference's order; the per-slot
  * tests they read, and the validation of the links, are computed first on
  * up to 16 host threads (the environment variable QV_HOST_THREADS sets the
- * number), which changes no result. */
+ * number; 0 evaluates every test inside the loop, without the pre-pass),
+ * which changes no result. */
 typedef struct qv_line_state {
   int32_t pcount;    /* slots in use (size(f)); pinsert may raise it */
   int32_t capacity;  /* slots allocated in every array below */

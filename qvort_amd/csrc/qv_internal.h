@@ -246,6 +246,13 @@ hipError_t launch_probe(Ctx &c, int nprobe, const double *d_y, double *d_u, doub
                         double *d_work,
                         bool tree, bool ref_scale, bool empty_core);
 
+// QV_HOST_THREADS=0: the line operations evaluate every slot's test in the
+// slot loop, without the pre-pass
+inline bool host_prepass() {
+  const char *e = std::getenv("QV_HOST_THREADS");
+  return !(e && std::atoi(e) == 0);
+}
+
 // host: f(lo, hi) over [0, n) on up to 16 threads, or QV_HOST_THREADS of
 // them when that is set (one thread for small n)
 template <class F>

@@ -142,14 +142,17 @@ struct Line {
     if (nact == 0) return QV_ERR_LINE_EMPTY;
     const int old = s.pcount;
     // each active slot's segment length (the test's input), in parallel
+    const bool prepass = host_prepass();
     std::vector<double> &pre = tl_pre;
     pre.resize(old);
-    par_for(old, [&](int lo, int hi) {
-      for (int i = lo; i < hi; ++i)
-        if (s.infront[i] != 0) pre[i] = lia_dist(X(i), GI(i));
-    });
-    // written by ghost_p (the ghosts, and the new point's position)
-    tl_dirty.assign((size_t)s.capacity, 0);
+    if (prepass)
+      par_for(old, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i)
+          if (s.infront[i] != 0) pre[i] = lia_dist(X(i), GI(i));
+      });
+    // written by ghost_p (the ghosts, and the new point's position); every
+    // slot marked without the pre-pass
+    tl_dirty.assign((size_t)s.capacity, prepass ? 0 : 1);
     dirty = tl_dirty.data();
     double total = 0.;
     // the first empty slot (minloc of infront): empty slots are only filled
@@ -257,14 +260,17 @@ struct Line {
 
   int premove() {
     const int n = s.pcount;
+    const bool prepass = host_prepass();
     std::vector<uint8_t> &hit = tl_flag;
     hit.resize(n);
-    par_for(n, [&](int lo, int hi) {
-      for (int i = lo; i < hi; ++i) hit[i] = s.infront[i] != 0 && premove_test(i);
-    });
+    if (prepass)
+      par_for(n, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) hit[i] = s.infront[i] != 0 && premove_test(i);
+      });
     // the test reads infront(i), infront(a), x(i) and x(aa): a change of
-    // any of them marks its slot (the link set below, clear)
-    tl_dirty.assign((size_t)n, 0);
+    // any of them marks its slot (the link set below, clear); every slot
+    // marked without the pre-pass
+    tl_dirty.assign((size_t)n, prepass ? 0 : 1);
     dirty = tl_dirty.data();
     for (int i = 0; i < n; ++i) {
       if (s.infront[i] == 0) continue;

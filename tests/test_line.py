@@ -183,15 +183,18 @@ def _big_state(seed):
                                       closestd=closestd)
 
 
-def test_threaded_pre_pass_changes_nothing(monkeypatch):
+@pytest.mark.parametrize("threads", ["0", "1"])
+def test_threaded_pre_pass_changes_nothing(monkeypatch, threads):
     """The line operations on a state past the threading threshold give the
-    same state and counters on one host thread and on eight."""
+    same state and counters with the tests computed beforehand on eight host
+    threads as with every test evaluated inside the slot loop ("0", the
+    plain sequential algorithm) or the pre-pass on one thread."""
     from qvort_amd import gpu, line
     p = gpu.params(velocity="Tree", tree_theta=0.4, boundary="periodic", box_size=0.2,
                    delta=5e-4, corea=1.3e-8, quant_circ=9.97e-4, dt=1e-6)
     runs = []
-    for threads in ("1", "8"):
-        monkeypatch.setenv("QV_HOST_THREADS", threads)
+    for t in (threads, "8"):
+        monkeypatch.setenv("QV_HOST_THREADS", t)
         st = _big_state(20261203)
         c = line.new_counts()
         line.pinsert(p, st, c)
