@@ -23,6 +23,9 @@
 //
 // dist = sqrt((dx^2 + dy^2) + dz^2) and width/dist in FP64 as the reference
 // forms them, so closest and closestd are bit-identical to the reference's.
+// width/dist < 0.5 is decided by a per-node threshold on d2, and a lane
+// leaves out subtrees farther than its best distance (neither changes a
+// result; see k_closest_tree).
 #include "qv_internal.h"
 
 #pragma clang fp contract(off)
@@ -40,6 +43,7 @@ struct CloseArgs {
   const double2 *geo;    // width
   const int32_t *count;  // points per node (root: all slots)
   const double *tclose;  // smallest d2 with width/dist < 0.5 (-1: exact test)
+  const double *corner;  // 3 per node: the cell's lower corner
   const int32_t *linf;
   const double *X, *Y, *Z;
   const int32_t *inf, *beh;
@@ -110,7 +114,21 @@ __global__ void __launch_bounds__(kBlock) k_closest_tree(CloseArgs A) {
         }
       }
     }
-    const uint64_t O = M & ~__ballot(taken);
+    // an opened node's subtree is left out for a lane whose best distance
+    // is below the distance to the node's cell box: every centroid below
+    // lies in the box (to rounding), so no test there can pass dist <
+    // closestd, and closestd only falls.  The margin 1e-6 on the square
+    // covers the rounding of the box distance and of the centroids; below
+    // bd = 1e-8 nothing is left out.
+    bool open_lane = mine && !taken;
+    if (open_lane && bd > 1e-8) {
+      const double *P = A.corner + 3 * (int64_t)n;
+      const double q0 = fmax(fmax(P[0] - x0, x0 - (P[0] + w)), 0.);
+      const double q1 = fmax(fmax(P[1] - x1, x1 - (P[1] + w)), 0.);
+      const double q2 = fmax(fmax(P[2] - x2, x2 - (P[2] + w)), 0.);
+      if ((q0 * q0 + q1 * q1) + q2 * q2 > (bd * bd) * 1.000001) open_lane = false;
+    }
+    const uint64_t O = __ballot(open_lane);
     if (O != 0 && n + 1 != skip) {
       // descend into the children [n+1, skip) with the opening lanes
       if (skip != e) {
@@ -193,6 +211,7 @@ hipError_t launch_closest(Ctx &c, bool tree) {
   A.geo = c.tree.geo;
   A.count = c.tree.count;
   A.tclose = c.tree.tclose;
+  A.corner = c.tree.corner;
   A.linf = c.tree.linf;
   A.X = c.x.x; A.Y = c.x.y; A.Z = c.x.z;
   A.inf = c.infront;
