@@ -215,10 +215,21 @@ struct NodeArgs {
 };
 
 // end of the level-`lev` cell whose first member is s: first t > s with
-// lcp(s, t) < lev (lcp(s, t) is non-increasing in t)
+// lcp(s, t) < lev (lcp(s, t) is non-increasing in t).  Most cells are a few
+// points long, so the search gallops from s (s+1, s+2, s+4, ...) before it
+// bisects: a few loads next to s instead of a bisection of all positions
 __device__ inline int cell_end(const NodeArgs &A, int s, int lev) {
   const uint64_t a0 = A.k0s[s], a1 = A.k1s[s];
   int lo = s + 1, hi = A.nact;  // answer in [lo, hi]
+  for (int64_t step = 1;; step <<= 1) {
+    const int64_t t = s + step;
+    if (t >= A.nact) break;
+    if (lcp(a0, a1, A.k0s[t], A.k1s[t]) < lev) {
+      hi = (int)t;
+      break;
+    }
+    lo = (int)t + 1;
+  }
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (lcp(a0, a1, A.k0s[mid], A.k1s[mid]) >= lev) lo = mid + 1;
